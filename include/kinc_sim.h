@@ -1,0 +1,146 @@
+/*
+ * kinc_sim.h — C-ABI of the MI355X pairwise Similarity engine.
+ *
+ * This is the drop-in boundary for KINC's Similarity analytic.  It replaces
+ * the per-block backend interface of the reference:
+ *   - Similarity::Serial::execute(const Block*) -> unique_ptr<Block>
+ *       (src/core/similarity_serial.h:12, src/core/similarity_serial.cpp:33-101)
+ *   - Similarity::OpenCL::Worker::execute(const Block*) -> unique_ptr<Block>
+ *       (src/core/similarity_opencl_worker.h:12, .cpp:93-271)
+ *   - Similarity::OpenCL::initialize(Context*) which uploads the expression
+ *       matrix once (src/core/similarity_opencl.cpp:34-70)
+ * A work block [start, start+count) of linear pair indices goes in; per pair
+ * the selected cluster count K, the per-sample labels and the per-cluster
+ * correlations come out, exactly the fields of Similarity::Pair
+ * (src/core/similarity.h:19-24).  Output buffers are caller-owned with fixed
+ * strides instead of a heap-allocated ResultBlock.  No C++ exceptions cross
+ * this boundary: every entry point returns 0 on success or a negative code,
+ * and kinc_sim_last_error() gives the message.
+ *
+ * One handle is bound to one device and one HIP stream and keeps the
+ * expression matrix resident in device memory.  A handle is not thread-safe:
+ * use one handle per host thread.
+ */
+#ifndef KINC_SIM_H
+#define KINC_SIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KINC_OK              0
+#define KINC_E_INVALID      -1   /* bad argument or parameter combination */
+#define KINC_E_DEVICE       -2   /* HIP runtime error */
+#define KINC_E_NOMEM        -3   /* device or host allocation failed */
+#define KINC_E_UNSUPPORTED  -4   /* valid in the reference, not built here */
+
+/* --clusmethod none|gmm|kmeans (src/core/similarity.h:39-44) */
+#define KINC_CLUS_NONE    0
+#define KINC_CLUS_GMM     1
+#define KINC_CLUS_KMEANS  2
+/* --corrmethod pearson|spearman (src/core/similarity.h:46-50) */
+#define KINC_CORR_PEARSON   0
+#define KINC_CORR_SPEARMAN  1
+/* --crit BIC|ICL (src/core/pairwise_clustering.h:9-13) */
+#define KINC_CRIT_BIC  0
+#define KINC_CRIT_ICL  1
+/* execution form of the device path */
+#define KINC_FORM_PLAIN       0  /* one launch per ksize sub-batch, one wave per pair */
+#define KINC_FORM_PERSISTENT  1  /* sample-count-binned queue drained by resident waves */
+
+/* Largest cluster count a pair may have (Pairwise::Index::MAX_CLUSTER_SIZE,
+ * src/core/pairwise_index.h:36). */
+#define KINC_MAX_CLUSTER_SIZE 64
+/* Pairs per work block (Similarity::size/makeWork, src/core/similarity.cpp:17-39). */
+#define KINC_WORK_BLOCK_SIZE 32768
+
+/*
+ * Analytic parameters.  Field meanings and defaults follow the reference's
+ * Similarity members (src/core/similarity.h:55-71) and CLI flags
+ * (src/core/similarity_input.cpp:96-254).
+ */
+typedef struct kinc_sim_params
+{
+   int32_t clus_method;    /* --clusmethod, default none */
+   int32_t corr_method;    /* --corrmethod, default pearson */
+   float   min_expr;       /* --minexpr, default -inf */
+   int32_t min_samples;    /* --minsamp, default 30 */
+   int32_t min_clusters;   /* --minclus, default 1 */
+   int32_t max_clusters;   /* --maxclus, default 5 */
+   int32_t criterion;      /* --crit, default ICL */
+   int32_t pre_out;        /* --preout, default 0 */
+   int32_t post_out;       /* --postout, default 0 */
+   float   min_corr;       /* --mincorr, default 0.5 (used by the CCM/CMX writer) */
+   float   max_corr;       /* --maxcorr, default 1.0 (used by the CCM/CMX writer) */
+   int32_t ksize;          /* --ksize, pairs per plain-form launch, default 4096 */
+   int32_t form;           /* KINC_FORM_*, default persistent */
+   int32_t reserved[5];    /* must be zero */
+} kinc_sim_params;
+
+typedef struct kinc_sim kinc_sim;
+
+/* Fill *p with the reference defaults. */
+void kinc_sim_default_params(kinc_sim_params* p);
+
+/*
+ * Create a handle on `device`, upload the expression matrix emx[G][S]
+ * (row-major float32, NaN = missing; the EMX payload layout of
+ * src/core/expressionmatrix.cpp:302-312) and keep it resident.
+ */
+int kinc_sim_create(const kinc_sim_params* params, const float* emx,
+                    int32_t gene_size, int32_t sample_size, int32_t device,
+                    kinc_sim** out);
+
+/*
+ * Execute one work block of `count` pairs starting at linear pair index
+ * `start` (pair k = x(x-1)/2 + y with x > y, src/core/pairwise_index.cpp:45-56).
+ * Host output buffers, written for every pair of the block:
+ *   K      [count]                   selected cluster count (0 = too few samples)
+ *   labels [count][sample_size]      per-sample labels (-9 NaN, -6 below minexpr,
+ *                                    -7/-8 pre/post outlier, >= 0 cluster);
+ *                                    may be NULL
+ *   corr   [count][max_clusters]     correlation of cluster k for k < K, NaN after
+ *   iters  [count][max_clusters]     optional (may be NULL): for the fit with k+1
+ *                                    clusters, the EM or Lloyd iteration count in
+ *                                    bits 0-11, bit 12 set if EM converged, bit 13
+ *                                    set if the fit failed; 0 if not run
+ * A block is the unit of the reference's stale-buffer semantics: the X
+ * workspace is shared by the pairs of one call in index order
+ * (src/core/similarity_serial.cpp:42), so results depend on the block split
+ * exactly as they do in the reference.  count <= KINC_WORK_BLOCK_SIZE.
+ */
+int kinc_sim_execute(kinc_sim* h, int64_t start, int64_t count,
+                     int8_t* K, int8_t* labels, float* corr, uint16_t* iters);
+
+/*
+ * Same as kinc_sim_execute but the outputs are device pointers on the
+ * handle's device and the call is asynchronous on `stream` (a hipStream_t,
+ * or NULL for the handle's own stream).  labels and iters may be NULL.
+ */
+int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count,
+                            int8_t* K, int8_t* labels, float* corr,
+                            uint16_t* iters, void* stream);
+
+/* Wait for all work queued by the handle. */
+int kinc_sim_synchronize(kinc_sim* h);
+
+/* Release the handle and its device memory. */
+void kinc_sim_destroy(kinc_sim* h);
+
+/* Message of the last failed call on this handle (or of the last failed
+ * kinc_sim_create when h is NULL). */
+const char* kinc_sim_last_error(const kinc_sim* h);
+
+/* Number of linear pairs for G genes: G(G-1)/2 (src/core/similarity.cpp:19). */
+int64_t kinc_pair_count(int32_t gene_size);
+
+/* Map a linear pair index to (x, y), x > y >= 0 (src/core/pairwise_index.cpp:30-56). */
+void kinc_pair_index(int64_t index, int32_t* x, int32_t* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,180 @@
+"""ctypes binding of the C-ABI declared in include/kinc_sim.h.
+
+The library is kinc_amd/libkinc.so, built in-tree by kinc_amd/build.py.
+There is no fallback: if the library is missing or fails to load, every
+entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkinc.so")
+
+KINC_OK = 0
+KINC_E_INVALID = -1
+KINC_E_DEVICE = -2
+KINC_E_NOMEM = -3
+KINC_E_UNSUPPORTED = -4
+
+CLUS = {"none": 0, "gmm": 1, "kmeans": 2}
+CORR = {"pearson": 0, "spearman": 1}
+CRIT = {"BIC": 0, "ICL": 1}
+FORM = {"plain": 0, "persistent": 1}
+
+WORK_BLOCK_SIZE = 32768
+MAX_CLUSTER_SIZE = 64
+
+
+class SimParams(ctypes.Structure):
+    """Mirror of kinc_sim_params (include/kinc_sim.h)."""
+    _fields_ = [
+        ("clus_method", ctypes.c_int32),
+        ("corr_method", ctypes.c_int32),
+        ("min_expr", ctypes.c_float),
+        ("min_samples", ctypes.c_int32),
+        ("min_clusters", ctypes.c_int32),
+        ("max_clusters", ctypes.c_int32),
+        ("criterion", ctypes.c_int32),
+        ("pre_out", ctypes.c_int32),
+        ("post_out", ctypes.c_int32),
+        ("min_corr", ctypes.c_float),
+        ("max_corr", ctypes.c_float),
+        ("ksize", ctypes.c_int32),
+        ("form", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
+    ]
+
+
+_lib = None
+
+# every entry point declared in include/kinc_sim.h: (name, restype, argtypes)
+_P = ctypes.c_void_p
+SIGNATURES = [
+    ("kinc_sim_default_params", None, [ctypes.POINTER(SimParams)]),
+    ("kinc_sim_create", ctypes.c_int, [ctypes.POINTER(SimParams), _P, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_P)]),
+    ("kinc_sim_execute", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P]),
+    ("kinc_sim_execute_device", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P, _P,
+                                               _P, _P, _P]),
+    ("kinc_sim_synchronize", ctypes.c_int, [_P]),
+    ("kinc_sim_destroy", None, [_P]),
+    ("kinc_sim_last_error", ctypes.c_char_p, [_P]),
+    ("kinc_pair_count", ctypes.c_int64, [ctypes.c_int32]),
+    ("kinc_pair_index", None, [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                               ctypes.POINTER(ctypes.c_int32)]),
+]
+
+
+def lib():
+    """Load libkinc.so once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("kinc_amd/libkinc.so is not built; run __graft_entry__.build()")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def default_params():
+    p = SimParams()
+    lib().kinc_sim_default_params(ctypes.byref(p))
+    return p
+
+
+def make_params(clusmethod="none", corrmethod="pearson", minexpr=float("-inf"), minsamp=30,
+                minclus=1, maxclus=5, crit="ICL", preout=False, postout=False, mincorr=0.5,
+                maxcorr=1.0, ksize=4096, form="persistent"):
+    """Build kinc_sim_params from the reference's CLI flag names and defaults
+    (src/core/similarity_input.cpp:96-254)."""
+    p = default_params()
+    p.clus_method = CLUS[clusmethod]
+    p.corr_method = CORR[corrmethod]
+    p.min_expr = minexpr
+    p.min_samples = minsamp
+    p.min_clusters = minclus
+    p.max_clusters = maxclus
+    p.criterion = CRIT[crit]
+    p.pre_out = int(bool(preout))
+    p.post_out = int(bool(postout))
+    p.min_corr = mincorr
+    p.max_corr = maxcorr
+    p.ksize = ksize
+    p.form = FORM[form]
+    return p
+
+
+class KincError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("kinc error %d: %s" % (code, msg))
+        self.code = code
+
+
+def pair_count(G):
+    return lib().kinc_pair_count(G)
+
+
+def pair_index(k):
+    x = ctypes.c_int32()
+    y = ctypes.c_int32()
+    lib().kinc_pair_index(k, ctypes.byref(x), ctypes.byref(y))
+    return x.value, y.value
+
+
+class Engine:
+    """One device handle with the expression matrix resident (kinc_sim_create)."""
+
+    def __init__(self, emx, params, device=0):
+        l = lib()
+        emx = np.ascontiguousarray(emx, dtype=np.float32)
+        self.G, self.S = emx.shape
+        self.params = params
+        h = _P()
+        rc = l.kinc_sim_create(ctypes.byref(params), emx.ctypes.data, self.G, self.S, device,
+                               ctypes.byref(h))
+        if rc != KINC_OK:
+            raise KincError(rc, l.kinc_sim_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != KINC_OK:
+            raise KincError(rc, lib().kinc_sim_last_error(self.h).decode())
+
+    def execute(self, start, count, want_labels=True, want_iters=True):
+        """Run one work block; returns (K, labels, corr, iters) numpy arrays."""
+        MC = self.params.max_clusters
+        K = np.empty(count, np.int8)
+        corr = np.empty((count, MC), np.float32)
+        labels = np.empty((count, self.S), np.int8) if want_labels else None
+        iters = np.empty((count, MC), np.uint16) if want_iters else None
+        rc = lib().kinc_sim_execute(self.h, start, count, K.ctypes.data,
+                                    labels.ctypes.data if labels is not None else None,
+                                    corr.ctypes.data,
+                                    iters.ctypes.data if iters is not None else None)
+        self._check(rc)
+        return K, labels, corr, iters
+
+    def execute_device(self, start, count, K_ptr, labels_ptr, corr_ptr, iters_ptr, stream=None):
+        rc = lib().kinc_sim_execute_device(self.h, start, count, K_ptr, labels_ptr, corr_ptr,
+                                           iters_ptr, stream)
+        self._check(rc)
+
+    def synchronize(self):
+        self._check(lib().kinc_sim_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().kinc_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

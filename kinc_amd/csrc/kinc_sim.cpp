@@ -1,0 +1,452 @@
+/*
+ * kinc_sim.cpp — host side of the C-ABI in include/kinc_sim.h.
+ *
+ * Replaces Similarity::OpenCL::initialize / Worker::execute
+ * (src/core/similarity_opencl.cpp:34-70, src/core/similarity_opencl_worker.cpp:93-271):
+ * the expression matrix is uploaded once and stays resident; a work block is
+ * executed either in the plain form (one launch per ksize sub-batch, one
+ * wave per pair, the register-slot count fixed by the sample count S) or in
+ * the persistent form (valid-sample counts first, pairs binned by the slot
+ * count their N needs, each bin drained by resident waves popping a device
+ * queue).  Both forms run the same per-pair device code and give identical
+ * results.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kinc_sim.h"
+#include "sim_device.h"
+
+extern "C" {
+const void* kinc_select_pair_kernel_k5(int P, int persistent);
+const void* kinc_select_pair_kernel_k8(int P, int persistent);
+const void* kinc_kernel_count_valid();
+const void* kinc_kernel_chunk_max();
+const void* kinc_kernel_bin_pairs();
+const void* kinc_kernel_zero_i32();
+}
+
+namespace {
+
+/* Register-slot counts that have a kernel instance (64*P samples per pair). */
+const int kSlotList[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16};
+const int kNumSlots = sizeof(kSlotList) / sizeof(kSlotList[0]);
+
+thread_local std::string g_create_error;
+
+int slots_for(int n)
+{
+   for ( int i = 0; i < kNumSlots; ++i )
+   {
+      if ( 64 * kSlotList[i] >= n ) return kSlotList[i];
+   }
+   return -1;
+}
+
+const void* kinc_select_pair_kernel(int P, int kmax, int persistent)
+{
+   if ( kmax == 5 ) return kinc_select_pair_kernel_k5(P, persistent);
+   if ( kmax == 8 ) return kinc_select_pair_kernel_k8(P, persistent);
+   return nullptr;
+}
+
+int next_pow2(int n)
+{
+   int p = 2;
+   while ( p < n ) p <<= 1;
+   return p;
+}
+
+size_t align16(size_t v)
+{
+   return (v + 15) & ~(size_t)15;
+}
+
+} // namespace
+
+struct kinc_sim
+{
+   kinc_sim_params p;
+   int device = 0;
+   int G = 0;
+   int S = 0;
+   int kmax = 5;
+   int num_cus = 0;
+   hipStream_t stream = nullptr;
+   float* d_emx = nullptr;
+   double* d_logN = nullptr;
+   int32_t* d_blockN = nullptr;
+   int32_t* d_chunkMax = nullptr;
+   int32_t* d_counters = nullptr;  /* [2*kNumSlots]: bin sizes, then pop heads */
+   int32_t* d_lists = nullptr;     /* [kNumSlots][KINC_WORK_BLOCK_SIZE] */
+   int32_t* d_binP = nullptr;
+   int nbins = 0;
+   /* staging for the host-buffer entry point */
+   int8_t* d_K = nullptr;
+   int8_t* d_labels = nullptr;
+   float* d_corr = nullptr;
+   uint16_t* d_iters = nullptr;
+   SimArgs base{};
+   size_t lds_bytes = 0;
+   std::string err;
+};
+
+static int fail(kinc_sim* h, int code, const std::string& msg)
+{
+   if ( h ) h->err = msg;
+   else g_create_error = msg;
+   return code;
+}
+
+#define HIP_TRY(h, call)                                                              \
+   do                                                                                 \
+   {                                                                                  \
+      hipError_t e_ = (call);                                                         \
+      if ( e_ != hipSuccess )                                                         \
+      {                                                                               \
+         return fail((h), KINC_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+      }                                                                               \
+   } while ( 0 )
+
+extern "C" void kinc_sim_default_params(kinc_sim_params* p)
+{
+   std::memset(p, 0, sizeof(*p));
+   p->clus_method = KINC_CLUS_NONE;
+   p->corr_method = KINC_CORR_PEARSON;
+   p->min_expr = -INFINITY;
+   p->min_samples = 30;
+   p->min_clusters = 1;
+   p->max_clusters = 5;
+   p->criterion = KINC_CRIT_ICL;
+   p->pre_out = 0;
+   p->post_out = 0;
+   p->min_corr = 0.5f;
+   p->max_corr = 1.0f;
+   p->ksize = 4096;
+   p->form = KINC_FORM_PERSISTENT;
+}
+
+extern "C" int64_t kinc_pair_count(int32_t G)
+{
+   return (int64_t)G * (G - 1) / 2;
+}
+
+extern "C" void kinc_pair_index(int64_t index, int32_t* x, int32_t* y)
+{
+   int64_t xx = (int64_t)((1.0 + std::sqrt(1.0 + 8.0 * (double)index)) / 2.0);
+   while ( xx * (xx - 1) / 2 > index ) --xx;
+   while ( (xx + 1) * xx / 2 <= index ) ++xx;
+   *x = (int32_t)xx;
+   *y = (int32_t)(index - xx * (xx - 1) / 2);
+}
+
+static int validate(const kinc_sim_params* p, int G, int S, std::string& why)
+{
+   if ( p->clus_method < 0 || p->clus_method > 2 ) { why = "clus_method must be 0 (none), 1 (gmm) or 2 (kmeans)"; return KINC_E_INVALID; }
+   if ( p->corr_method < 0 || p->corr_method > 1 ) { why = "corr_method must be 0 (pearson) or 1 (spearman)"; return KINC_E_INVALID; }
+   if ( p->criterion < 0 || p->criterion > 1 ) { why = "criterion must be 0 (BIC) or 1 (ICL)"; return KINC_E_INVALID; }
+   if ( p->min_samples < 1 ) { why = "min_samples must be >= 1"; return KINC_E_INVALID; }
+   if ( p->min_clusters < 1 || p->max_clusters < 1 || p->max_clusters > KINC_MAX_CLUSTER_SIZE )
+   {
+      why = "cluster range must lie in [1, 64]";
+      return KINC_E_INVALID;
+   }
+   if ( p->max_clusters < p->min_clusters )
+   {
+      /* similarity.cpp:181-187 */
+      why = "Minimum clusters must be less than or equal to maximum clusters.";
+      return KINC_E_INVALID;
+   }
+   if ( p->ksize < 1 ) { why = "ksize must be >= 1"; return KINC_E_INVALID; }
+   if ( p->form != KINC_FORM_PLAIN && p->form != KINC_FORM_PERSISTENT ) { why = "form must be 0 (plain) or 1 (persistent)"; return KINC_E_INVALID; }
+   for ( int r : p->reserved )
+   {
+      if ( r != 0 ) { why = "reserved fields must be zero"; return KINC_E_INVALID; }
+   }
+   if ( G < 2 || S < 1 ) { why = "need at least 2 genes and 1 sample"; return KINC_E_INVALID; }
+   if ( p->max_clusters > 8 )
+   {
+      why = "max_clusters > 8 is not built on the device path";
+      return KINC_E_UNSUPPORTED;
+   }
+   if ( S > KINC_MAX_SAMPLES )
+   {
+      why = "sample_size > " + std::to_string(KINC_MAX_SAMPLES) + " is not built on the device path";
+      return KINC_E_UNSUPPORTED;
+   }
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
+                               int32_t G, int32_t S, int32_t device, kinc_sim** out)
+{
+   if ( !params || !emx || !out ) return fail(nullptr, KINC_E_INVALID, "null argument");
+   *out = nullptr;
+   std::string why;
+   int rc = validate(params, G, S, why);
+   if ( rc != KINC_OK ) return fail(nullptr, rc, why);
+
+   kinc_sim* h = new kinc_sim();
+   h->p = *params;
+   h->device = device;
+   h->G = G;
+   h->S = S;
+   h->kmax = params->max_clusters <= 5 ? 5 : 8;
+
+   auto bail = [&](int code) {
+      g_create_error = h->err;
+      kinc_sim_destroy(h);
+      return code;
+   };
+#define CREATE_TRY(call)                                                             \
+   do                                                                                \
+   {                                                                                 \
+      hipError_t e_ = (call);                                                        \
+      if ( e_ != hipSuccess )                                                        \
+      {                                                                              \
+         h->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+         return bail(KINC_E_DEVICE);                                                 \
+      }                                                                              \
+   } while ( 0 )
+
+   CREATE_TRY(hipSetDevice(device));
+   hipDeviceProp_t prop;
+   CREATE_TRY(hipGetDeviceProperties(&prop, device));
+   h->num_cus = prop.multiProcessorCount;
+   CREATE_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+   const size_t emx_bytes = (size_t)G * (size_t)S * sizeof(float);
+   CREATE_TRY(hipMalloc(&h->d_emx, emx_bytes));
+   CREATE_TRY(hipMemcpy(h->d_emx, emx, emx_bytes, hipMemcpyHostToDevice));
+
+   /* log(N) of the BIC term (pairwise_clustering.cpp:160), evaluated by the
+    * host libm once per possible N */
+   std::vector<double> logN((size_t)S + 1);
+   for ( int n = 0; n <= S; ++n ) logN[(size_t)n] = std::log((double)n);
+   CREATE_TRY(hipMalloc(&h->d_logN, sizeof(double) * logN.size()));
+   CREATE_TRY(hipMemcpy(h->d_logN, logN.data(), sizeof(double) * logN.size(), hipMemcpyHostToDevice));
+
+   const size_t B = KINC_WORK_BLOCK_SIZE;
+   CREATE_TRY(hipMalloc(&h->d_blockN, sizeof(int32_t) * B));
+   CREATE_TRY(hipMalloc(&h->d_chunkMax, sizeof(int32_t) * (B / 64 + 1)));
+   CREATE_TRY(hipMalloc(&h->d_counters, sizeof(int32_t) * 2 * kNumSlots));
+   CREATE_TRY(hipMalloc(&h->d_lists, sizeof(int32_t) * B * kNumSlots));
+
+   /* bins: every slot count up to the one S needs */
+   std::vector<int32_t> binP;
+   const int Pmax = slots_for(S);
+   for ( int i = 0; i < kNumSlots && kSlotList[i] <= Pmax; ++i ) binP.push_back(kSlotList[i]);
+   h->nbins = (int)binP.size();
+   CREATE_TRY(hipMalloc(&h->d_binP, sizeof(int32_t) * binP.size()));
+   CREATE_TRY(hipMemcpy(h->d_binP, binP.data(), sizeof(int32_t) * binP.size(), hipMemcpyHostToDevice));
+
+   /* LDS carve-up */
+   SimArgs& a = h->base;
+   a.emx = h->d_emx;
+   a.G = G;
+   a.S = S;
+   a.clus = params->clus_method;
+   a.corr = params->corr_method;
+   a.min_expr = params->min_expr;
+   a.min_samples = params->min_samples;
+   a.min_k = params->min_clusters;
+   a.max_k = params->max_clusters;
+   a.crit = params->criterion;
+   a.pre_out = params->pre_out ? 1 : 0;
+   a.post_out = params->post_out ? 1 : 0;
+   a.logN = h->d_logN;
+   a.blockN = h->d_blockN;
+   a.chunkMax = h->d_chunkMax;
+   a.np2 = next_pow2(S);
+   a.tile_stride = 64 + 4;
+   size_t off = 0;
+   a.lds_x = (int32_t)off;
+   off = align16(off + sizeof(float) * 2 * (size_t)S);
+   a.lds_lab = (int32_t)off;
+   off = align16(off + (size_t)S);
+   a.lds_w = (int32_t)off;
+   off = align16(off + (size_t)S);
+   a.lds_u = (int32_t)off;
+   const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
+   const size_t sort_bytes = sizeof(float) * 3 * (size_t)a.np2;
+   off += std::max(tile_bytes, sort_bytes);
+   h->lds_bytes = align16(off);
+
+   *out = h;
+   return KINC_OK;
+#undef CREATE_TRY
+}
+
+static int ensure_staging(kinc_sim* h)
+{
+   if ( h->d_K ) return KINC_OK;
+   const size_t B = KINC_WORK_BLOCK_SIZE;
+   HIP_TRY(h, hipMalloc(&h->d_K, B));
+   HIP_TRY(h, hipMalloc(&h->d_labels, B * (size_t)h->S));
+   HIP_TRY(h, hipMalloc(&h->d_corr, sizeof(float) * B * (size_t)h->p.max_clusters));
+   HIP_TRY(h, hipMalloc(&h->d_iters, sizeof(uint16_t) * B * (size_t)h->p.max_clusters));
+   return KINC_OK;
+}
+
+static int launch(kinc_sim* h, const void* fn, unsigned grid, unsigned block, void** args,
+                  size_t lds, hipStream_t s)
+{
+   HIP_TRY(h, hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, s));
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count,
+                                       int8_t* K, int8_t* labels, float* corr,
+                                       uint16_t* iters, void* stream)
+{
+   if ( !h ) return KINC_E_INVALID;
+   if ( count < 0 || count > KINC_WORK_BLOCK_SIZE ) return fail(h, KINC_E_INVALID, "count must lie in [0, 32768]");
+   if ( start < 0 || start + count > kinc_pair_count(h->G) ) return fail(h, KINC_E_INVALID, "pair range outside the matrix");
+   if ( count == 0 ) return KINC_OK;
+   if ( !K || !corr ) return fail(h, KINC_E_INVALID, "K and corr outputs are required");
+   HIP_TRY(h, hipSetDevice(h->device));
+   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+   const int cnt = (int)count;
+
+   SimArgs a = h->base;
+   a.block_start = start;
+   a.block_count = cnt;
+   a.K_out = K;
+   a.labels_out = labels;
+   a.corr_out = corr;
+   a.iters_out = iters;
+
+   const bool need_counts = (h->p.form == KINC_FORM_PERSISTENT) || (h->p.corr_method == KINC_CORR_PEARSON);
+   if ( need_counts )
+   {
+      const float* emx = h->d_emx;
+      int S = h->S;
+      int64_t bs = start;
+      float me = h->p.min_expr;
+      int32_t* outN = h->d_blockN;
+      void* args[] = {&emx, &S, &bs, (void*)&cnt, &me, &outN};
+      int rc = launch(h, kinc_kernel_count_valid(), (unsigned)((cnt + 3) / 4), 256, args, 0, s);
+      if ( rc ) return rc;
+   }
+   if ( h->p.corr_method == KINC_CORR_PEARSON )
+   {
+      const int32_t* Np = h->d_blockN;
+      int32_t* cm = h->d_chunkMax;
+      void* args[] = {&Np, (void*)&cnt, &cm};
+      const int chunks = (cnt + 63) / 64;
+      int rc = launch(h, kinc_kernel_chunk_max(), (unsigned)((chunks + 255) / 256), 256, args, 0, s);
+      if ( rc ) return rc;
+   }
+
+   if ( h->p.form == KINC_FORM_PLAIN )
+   {
+      const int P = slots_for(h->S);
+      const void* fn = kinc_select_pair_kernel(P, h->kmax, 0);
+      if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
+      for ( int first = 0; first < cnt; first += h->p.ksize )
+      {
+         a.first = first;
+         a.launch_count = std::min(h->p.ksize, cnt - first);
+         void* args[] = {&a};
+         int rc = launch(h, fn, (unsigned)a.launch_count, 64, args, h->lds_bytes, s);
+         if ( rc ) return rc;
+      }
+      return KINC_OK;
+   }
+
+   /* persistent: bin by slot count, then drain each bin with resident waves */
+   {
+      int32_t* ctr = h->d_counters;
+      int n = 2 * kNumSlots;
+      void* args[] = {&ctr, &n};
+      int rc = launch(h, kinc_kernel_zero_i32(), 1, 64, args, 0, s);
+      if ( rc ) return rc;
+   }
+   {
+      const int32_t* Np = h->d_blockN;
+      const int32_t* bp = h->d_binP;
+      int nb = h->nbins;
+      int32_t* ctr = h->d_counters;
+      int32_t* lists = h->d_lists;
+      void* args[] = {&Np, (void*)&cnt, &bp, &nb, &ctr, &lists};
+      int rc = launch(h, kinc_kernel_bin_pairs(), (unsigned)((cnt + 255) / 256), 256, args, 0, s);
+      if ( rc ) return rc;
+   }
+   for ( int b = h->nbins - 1; b >= 0; --b )
+   {
+      const int P = kSlotList[b];
+      const void* fn = kinc_select_pair_kernel(P, h->kmax, 1);
+      if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
+      int per_cu = 0;
+      HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
+      if ( per_cu < 1 ) per_cu = 1;
+      long long waves = (long long)per_cu * h->num_cus;
+      long long grid = std::min<long long>(waves, (long long)cnt);
+      a.list = h->d_lists + (size_t)b * (size_t)cnt;
+      a.list_len = h->d_counters + b;
+      a.list_head = h->d_counters + kNumSlots + b;
+      void* args[] = {&a};
+      int rc = launch(h, fn, (unsigned)grid, 64, args, h->lds_bytes, s);
+      if ( rc ) return rc;
+   }
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_execute(kinc_sim* h, int64_t start, int64_t count,
+                                int8_t* K, int8_t* labels, float* corr, uint16_t* iters)
+{
+   if ( !h ) return KINC_E_INVALID;
+   if ( count == 0 ) return KINC_OK;
+   if ( !K || !corr ) return fail(h, KINC_E_INVALID, "K and corr outputs are required");
+   int rc = ensure_staging(h);
+   if ( rc ) return rc;
+   rc = kinc_sim_execute_device(h, start, count, h->d_K, labels ? h->d_labels : nullptr, h->d_corr,
+                                iters ? h->d_iters : nullptr, nullptr);
+   if ( rc ) return rc;
+   const size_t n = (size_t)count;
+   const size_t MC = (size_t)h->p.max_clusters;
+   HIP_TRY(h, hipMemcpyAsync(K, h->d_K, n, hipMemcpyDeviceToHost, h->stream));
+   HIP_TRY(h, hipMemcpyAsync(corr, h->d_corr, sizeof(float) * n * MC, hipMemcpyDeviceToHost, h->stream));
+   if ( labels ) HIP_TRY(h, hipMemcpyAsync(labels, h->d_labels, n * (size_t)h->S, hipMemcpyDeviceToHost, h->stream));
+   if ( iters ) HIP_TRY(h, hipMemcpyAsync(iters, h->d_iters, sizeof(uint16_t) * n * MC, hipMemcpyDeviceToHost, h->stream));
+   HIP_TRY(h, hipStreamSynchronize(h->stream));
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_synchronize(kinc_sim* h)
+{
+   if ( !h ) return KINC_E_INVALID;
+   HIP_TRY(h, hipStreamSynchronize(h->stream));
+   return KINC_OK;
+}
+
+extern "C" void kinc_sim_destroy(kinc_sim* h)
+{
+   if ( !h ) return;
+   (void)hipSetDevice(h->device);
+   if ( h->stream ) (void)hipStreamSynchronize(h->stream);
+   (void)hipFree(h->d_emx);
+   (void)hipFree(h->d_logN);
+   (void)hipFree(h->d_blockN);
+   (void)hipFree(h->d_chunkMax);
+   (void)hipFree(h->d_counters);
+   (void)hipFree(h->d_lists);
+   (void)hipFree(h->d_binP);
+   (void)hipFree(h->d_K);
+   (void)hipFree(h->d_labels);
+   (void)hipFree(h->d_corr);
+   (void)hipFree(h->d_iters);
+   if ( h->stream ) (void)hipStreamDestroy(h->stream);
+   delete h;
+}
+
+extern "C" const char* kinc_sim_last_error(const kinc_sim* h)
+{
+   return h ? h->err.c_str() : g_create_error.c_str();
+}

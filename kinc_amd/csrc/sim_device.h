@@ -1,0 +1,61 @@
+/*
+ * sim_device.h — argument block shared by the host launcher and the HIP
+ * kernels of the pairwise Similarity path.
+ */
+#ifndef KINC_SIM_DEVICE_H
+#define KINC_SIM_DEVICE_H
+
+#include <stdint.h>
+
+/* 2*log(2*pi) in double: the D*log(2.0f*M_PI) term of the GMM normaliser
+ * (src/core/pairwise_gmm.cpp:46, D = 2). */
+#define KINC_TWO_LOG_2PI 3.6757541328186907
+
+/* Points held per lane in registers: a pair with N valid samples runs on the
+ * kernel instance with 64*P >= N. */
+#define KINC_MAX_P 16
+#define KINC_MAX_SAMPLES (64 * KINC_MAX_P)
+
+/* Pair-kernel argument block (passed by value). */
+struct SimArgs
+{
+   const float* emx;         /* [G][S] resident expression matrix */
+   int32_t G;
+   int32_t S;
+   int64_t block_start;      /* linear index of block-relative pair 0 */
+   int32_t block_count;      /* pairs in the block */
+   int32_t first;            /* plain form: first block-relative pair of the launch */
+   int32_t launch_count;     /* plain form: pairs in the launch */
+   const int32_t* list;      /* persistent form: block-relative pair ids of one bin */
+   int32_t* list_head;       /* persistent form: atomic pop counter of the bin */
+   const int32_t* list_len;  /* persistent form: number of ids in the bin */
+
+   int32_t clus;
+   int32_t corr;
+   float min_expr;
+   int32_t min_samples;
+   int32_t min_k;
+   int32_t max_k;
+   int32_t crit;
+   int32_t pre_out;
+   int32_t post_out;
+
+   const double* logN;       /* log((double)n) for n in [0, S] */
+   const int32_t* blockN;    /* valid-sample count of every pair of the block */
+   const int32_t* chunkMax;  /* max of blockN over 64-pair chunks */
+
+   int8_t* K_out;            /* [block_count] */
+   int8_t* labels_out;       /* [block_count][S] or null */
+   float* corr_out;          /* [block_count][max_k] */
+   uint16_t* iters_out;      /* [block_count][max_k] or null */
+
+   /* LDS carve-up (byte offsets in the dynamic shared segment) */
+   int32_t lds_x;            /* float2[S]  compact X, then the stale-read view */
+   int32_t lds_lab;          /* int8[S]    full-index labels */
+   int32_t lds_w;            /* int8[S]    work labels of the current fit */
+   int32_t lds_u;            /* union: chain-term tile / sort buffers */
+   int32_t np2;              /* next power of two >= S (min 2) */
+   int32_t tile_stride;      /* floats per chain row of the term tile */
+};
+
+#endif

@@ -1,0 +1,84 @@
+/*
+ * sim_util_kernels.hip — block preparation kernels of the pairwise
+ * Similarity path: valid-sample counts, chunk maxima for the stale-read
+ * view, sample-count binning for the persistent form, and a zero fill.
+ */
+#include "sim_pair_impl.h"
+
+/* Valid-sample count per pair of a block (one wave per pair, 4 per group). */
+__global__ __launch_bounds__(256) void k_count_valid(const float* emx, int S, int64_t block_start,
+                                                     int count, float min_expr, int32_t* outN)
+{
+   const int w = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+   if ( w >= count ) return;
+   int gx, gy;
+   pair_xy(block_start + w, gx, gy);
+   const float* r1 = emx + (size_t)gx * (size_t)S;
+   const float* r2 = emx + (size_t)gy * (size_t)S;
+   int n = 0;
+   for ( int s0 = 0; s0 < S; s0 += WAVE )
+   {
+      int s = s0 + lane_id();
+      bool valid = false;
+      if ( s < S )
+      {
+         float g1 = r1[s], g2 = r2[s];
+         valid = !isnan(g1) && !isnan(g2) && !(g1 < min_expr || g2 < min_expr);
+      }
+      n += popc(ballot(valid));
+   }
+   if ( lane_id() == 0 ) outN[w] = n;
+}
+
+/* max of N over 64-pair chunks */
+__global__ __launch_bounds__(256) void k_chunk_max(const int32_t* N, int count, int32_t* cmax)
+{
+   const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+   const int lo = c * WAVE;
+   if ( lo >= count ) return;
+   int hi = lo + WAVE;
+   hi = hi < count ? hi : count;
+   int m = -1;
+   for ( int i = lo; i < hi; ++i ) m = N[i] > m ? N[i] : m;
+   cmax[c] = m;
+}
+
+/*
+ * Bin the pairs of a block by register slots P(N) for the persistent form.
+ * Bin b holds pairs whose N needs the kernel instance with slot count
+ * binP[b]; pairs below min_samples (no clustering to do) go to the
+ * smallest bin that still fits their N.  counters[nb] must be zero.
+ */
+__global__ __launch_bounds__(256) void k_bin_pairs(const int32_t* N, int count, const int32_t* binP,
+                                                   int nb, int32_t* counters, int32_t* lists)
+{
+   const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+   if ( q >= count ) return;
+   const int n = N[q];
+   int b = nb - 1;
+   for ( int i = 0; i < nb; ++i )
+   {
+      if ( n <= 64 * binP[i] )
+      {
+         b = i;
+         break;
+      }
+   }
+   const int pos = atomicAdd(&counters[b], 1);
+   lists[(size_t)b * count + pos] = q;
+}
+
+/* Zero a small int array (used instead of a memset node). */
+__global__ void k_zero_i32(int32_t* p, int n)
+{
+   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+   if ( i < n ) p[i] = 0;
+}
+
+
+extern "C" {
+const void* kinc_kernel_count_valid() { return (const void*)&k_count_valid; }
+const void* kinc_kernel_chunk_max() { return (const void*)&k_chunk_max; }
+const void* kinc_kernel_bin_pairs() { return (const void*)&k_bin_pairs; }
+const void* kinc_kernel_zero_i32() { return (const void*)&k_zero_i32; }
+}

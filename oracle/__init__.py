@@ -1,0 +1,160 @@
+"""CPU oracle of the pairwise Similarity path — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package, and only as the checker or the timed CPU baseline.
+The product (kinc_amd/) never imports, links or calls it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkinc_oracle.so")
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        l = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32, i64, f32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+        sig = {
+            "oracle_pair_index": (None, [i64, P, P]),
+            "oracle_pair_index_loop": (None, [i64, P, P]),
+            "oracle_lcg": (ctypes.c_int, [P]),
+            "oracle_expf": (f32, [f32]),
+            "oracle_logf": (f32, [f32]),
+            "oracle_bic": (f32, [ctypes.c_int, f32, ctypes.c_int, ctypes.c_int]),
+            "oracle_icl": (f32, [ctypes.c_int, f32, ctypes.c_int, ctypes.c_int, f32]),
+            "oracle_gmm_fit_flat": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P, P]),
+            "oracle_kmeans_fit_flat": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P]),
+            "oracle_mark_outliers_flat": (None, [P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int,
+                                                 ctypes.c_int]),
+            "oracle_pearson_flat": (f32, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "oracle_spearman_flat": (f32, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "oracle_bitonic_sort": (None, [ctypes.c_int, P, P]),
+            "oracle_execute_block": (ctypes.c_int, [P, P, i32, i32, i64, i64, P, P, P, P]),
+            "oracle_corr_fp64_block": (ctypes.c_int, [P, P, i32, i32, i64, i64, P, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+def execute_block(params, emx, start, count, want_labels=True):
+    """Serial restatement of Similarity::Serial::execute for one block."""
+    emx = np.ascontiguousarray(emx, np.float32)
+    G, S = emx.shape
+    MC = params.max_clusters
+    K = np.empty(count, np.int8)
+    labels = np.empty((count, S), np.int8) if want_labels else None
+    corr = np.empty((count, MC), np.float32)
+    iters = np.empty((count, MC), np.uint16)
+    rc = lib().oracle_execute_block(ctypes.byref(params), emx.ctypes.data, G, S, start, count,
+                                    K.ctypes.data, _ptr(labels), corr.ctypes.data,
+                                    iters.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle_execute_block failed: %d" % rc)
+    return K, labels, corr, iters
+
+
+def corr_fp64_block(params, emx, start, count, K, labels):
+    emx = np.ascontiguousarray(emx, np.float32)
+    G, S = emx.shape
+    out = np.empty((count, params.max_clusters), np.float64)
+    lib().oracle_corr_fp64_block(ctypes.byref(params), emx.ctypes.data, G, S, start, count,
+                                 np.ascontiguousarray(K).ctypes.data,
+                                 np.ascontiguousarray(labels).ctypes.data, out.ctypes.data)
+    return out
+
+
+def pair_index(k, loop=False):
+    x = ctypes.c_int32()
+    y = ctypes.c_int32()
+    fn = lib().oracle_pair_index_loop if loop else lib().oracle_pair_index
+    fn(k, ctypes.byref(x), ctypes.byref(y))
+    return x.value, y.value
+
+
+def lcg_sequence(n, seed=1):
+    st = ctypes.c_uint64(seed)
+    return [lib().oracle_lcg(ctypes.byref(st)) for _ in range(n)]
+
+
+def expf(x):
+    return lib().oracle_expf(x)
+
+
+def logf(x):
+    return lib().oracle_logf(x)
+
+
+def gmm_fit(X, K):
+    X = np.ascontiguousarray(X, np.float32)
+    N = X.shape[0]
+    labels = np.zeros(N, np.int8)
+    logL = ctypes.c_float()
+    E = ctypes.c_float()
+    st = np.zeros(4, np.int32)
+    ok = lib().oracle_gmm_fit_flat(X.ctypes.data, N, K, labels.ctypes.data, ctypes.byref(logL),
+                                   ctypes.byref(E), st.ctypes.data)
+    return bool(ok), labels, logL.value, E.value, st
+
+
+def kmeans_fit(X, K):
+    X = np.ascontiguousarray(X, np.float32)
+    N = X.shape[0]
+    labels = np.zeros(N, np.int8)
+    logL = ctypes.c_float()
+    st = np.zeros(4, np.int32)
+    ok = lib().oracle_kmeans_fit_flat(X.ctypes.data, N, K, labels.ctypes.data,
+                                      ctypes.byref(logL), st.ctypes.data)
+    return bool(ok), labels, logL.value, st
+
+
+def mark_outliers(X, labels, j, cluster, marker):
+    X = np.ascontiguousarray(X, np.float32)
+    lab = np.ascontiguousarray(labels, np.int8).copy()
+    lib().oracle_mark_outliers_flat(X.ctypes.data, X.shape[0], j, lab.ctypes.data, cluster, marker)
+    return lab
+
+
+def pearson(data, labels, cluster, minsamp):
+    data = np.ascontiguousarray(data, np.float32)
+    labels = np.ascontiguousarray(labels, np.int8)
+    return lib().oracle_pearson_flat(data.ctypes.data, labels.ctypes.data, labels.shape[0],
+                                     cluster, minsamp)
+
+
+def spearman(data, labels, cluster, minsamp):
+    data = np.ascontiguousarray(data, np.float32)
+    labels = np.ascontiguousarray(labels, np.int8)
+    return lib().oracle_spearman_flat(data.ctypes.data, labels.ctypes.data, labels.shape[0],
+                                      cluster, minsamp)
+
+
+def bitonic_sort(key, payload):
+    key = np.ascontiguousarray(key, np.float32).copy()
+    payload = np.ascontiguousarray(payload, np.float32).copy()
+    lib().oracle_bitonic_sort(key.shape[0], key.ctypes.data, payload.ctypes.data)
+    return key, payload
+
+
+def bic(K, logL, N, D=2):
+    return lib().oracle_bic(K, logL, N, D)
