@@ -1,0 +1,270 @@
+"""Benchmark of the pairwise Similarity path on one MI355X.
+
+Workload (the headline configuration): a synthetic 20,000-gene x 500-sample
+FP32 expression matrix from the module generator (10% NaN, seed 3), KINC
+Similarity with GMM clustering (K = 1..5 by BIC), pre- and post-clustering
+outlier removal, minimum 30 samples and Spearman correlation.  One step is
+one KINC work block of 32,768 consecutive gene pairs; the steps sample the
+199,990,000-pair space at every 97th block.  The expression matrix is
+resident in HBM before timing starts and outputs stay on the device.
+
+The JSON line reports pairs/s of the batched persistent form (`value`), of
+the plain one-launch-per-ksize form (`plain_value`), the FP32 VALU roofline
+fraction of the dominant pair kernel, timed with HIP events on the stream it
+runs on, and the CPU oracle timed on a bounded sample of the same pairs on
+this host (which also re-checks bit-exact parity on that sample).
+
+Usage: python bench.py [--gpus 1] [--steps 8] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "gene pairs/sec GMM+Spearman 20k×500, batched vs plain; p50/p99 query latency"
+FP32_PEAK_TFLOPS = 157.3
+BLOCK = 32768
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pair_flops(N, iters, corr_method, labels, K):
+    """Algorithmic FP32 flops of one pair from its per-fit iteration records.
+
+    GMM fit with K components on N points, T E-steps and T_M M-steps:
+    N*K*(18*T + 20*T_M) + 2*N*T (every add, sub, mul, div counts 1).
+    Pearson adds 8 flops per in-cluster point; Spearman adds none (integer
+    ranks).
+    """
+    total = 0
+    for k in range(iters.shape[0]):
+        it = int(iters[k])
+        if it == 0:
+            continue
+        T = it & 0xFFF
+        conv = (it >> 12) & 1
+        TM = T - 1 if conv else T
+        Kc = k + 1
+        total += N * Kc * (18 * T + 20 * TM) + 2 * N * T
+    if corr_method == "pearson":
+        for k in range(int(K)):
+            total += 8 * int(np.count_nonzero(labels == k))
+    return total
+
+
+def block_flops(K, labels, iters, corr_method):
+    N = labels.shape[1] - np.count_nonzero((labels == -9) | (labels == -6), axis=1)
+    f = np.zeros(K.shape[0], np.float64)
+    for q in range(K.shape[0]):
+        f[q] = pair_flops(int(N[q]), iters[q], corr_method, labels[q], K[q])
+    return f, N
+
+
+def slots_for(n):
+    for p in (1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16):
+        if 64 * p >= n:
+            return p
+    return -1
+
+
+def run_form(native, x, flags, form, blocks, warmup, torch):
+    import ctypes  # noqa: F401
+    G, S = x.shape
+    total_pairs = native.pair_count(G)
+    p = native.make_params(form=form, **flags)
+    eng = native.Engine(x, p)
+    MC = p.max_clusters
+    nsteps = len(blocks)
+    dev = torch.device("cuda:0")
+    dK = torch.empty((nsteps, BLOCK), dtype=torch.int8, device=dev)
+    dL = torch.empty((nsteps, BLOCK, S), dtype=torch.int8, device=dev)
+    dC = torch.empty((nsteps, BLOCK, MC), dtype=torch.float32, device=dev)
+    dI = torch.empty((nsteps, BLOCK, MC), dtype=torch.int16, device=dev)
+
+    def run(i, b):
+        start = b * BLOCK
+        count = min(BLOCK, total_pairs - start)
+        eng.execute_device(start, count, dK[i].data_ptr(), dL[i].data_ptr(), dC[i].data_ptr(),
+                           dI[i].data_ptr())
+        return count
+
+    for i in range(warmup):
+        run(i % nsteps, blocks[i % nsteps])
+    eng.synchronize()
+    torch.cuda.synchronize()
+    eng.set_profiling(True)
+    eng.profile_read()
+    t0 = time.perf_counter()
+    pairs = 0
+    for i, b in enumerate(blocks):
+        pairs += run(i, b)
+        log("  %s step %d/%d queued (block %d)" % (form, i + 1, nsteps, b))
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = eng.profile_read()
+    eng.set_profiling(False)
+    out = dict(
+        K=dK.cpu().numpy(), labels=dL.cpu().numpy(), corr=dC.cpu().numpy(),
+        iters=dI.cpu().numpy().view(np.uint16))
+    eng.close()
+    return pairs, t1 - t0, prof, out
+
+
+def cpu_baseline(oracle, native, x, flags, blocks, seconds, gpu_out):
+    """Oracle (single thread) on the first pairs of the timed blocks, up to
+    `seconds` of CPU time; returns pairs/s and the bit-exact check."""
+    from parity_util import count_mismatching_pairs as compare_block
+    p = native.make_params(**flags)
+    G, S = x.shape
+    done = 0
+    t_used = 0.0
+    per_block = 24
+    mismatches = 0
+    checked = 0
+    rounds = 0
+    while t_used < seconds and rounds < 64:
+        for i, b in enumerate(blocks):
+            lo = rounds * per_block
+            if lo >= BLOCK:
+                break
+            start = b * BLOCK + lo
+            t0 = time.process_time()
+            want = oracle.execute_block(p, x, start, per_block)
+            t_used += time.process_time() - t0
+            done += per_block
+            got = tuple(a[i][lo:lo + per_block] for a in
+                        (gpu_out["K"], gpu_out["labels"], gpu_out["corr"], gpu_out["iters"]))
+            mismatches += compare_block(got, want)
+            checked += per_block
+            if t_used >= seconds:
+                break
+        rounds += 1
+    return done / t_used, done, t_used, checked, mismatches
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; the pair space would shard "
+              "across GPUs by contiguous work-block ranges (see the sharding notes); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from kinc_amd import _native as native, gemgen
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    native.lib()
+
+    cfg = gemgen.CONFIGS["H"]
+    flags = dict(cfg["params"])
+    t = time.time()
+    x = gemgen.make("H")
+    G, S = x.shape
+    log("generated %dx%d GEM in %.1fs" % (G, S, time.time() - t))
+    total_pairs = native.pair_count(G)
+    nblocks = (total_pairs + BLOCK - 1) // BLOCK
+    blocks = [(97 * (i + 1)) % nblocks for i in range(args.steps)]
+
+    pairs, secs, prof, out = run_form(native, x, flags, "persistent", blocks, args.warmup, torch)
+    value = pairs / secs
+    log("persistent: %d pairs in %.3fs -> %.0f pairs/s; kernels %s" % (pairs, secs, value, prof))
+
+    plain_value = None
+    if not args.no_plain:
+        pp, ps, pprof, pout = run_form(native, x, flags, "plain", blocks, args.warmup, torch)
+        plain_value = pp / ps
+        log("plain: %d pairs in %.3fs -> %.0f pairs/s" % (pp, ps, plain_value))
+        same = all(np.array_equal(out[k], pout[k]) for k in ("K", "labels", "iters")) and \
+            np.array_equal(np.nan_to_num(out["corr"], nan=9.0).view(np.uint32),
+                           np.nan_to_num(pout["corr"], nan=9.0).view(np.uint32))
+        log("plain == persistent (bit-exact):", same)
+
+    # algorithmic flops per step and per pair class
+    flops_total = 0.0
+    flops_by_P = {}
+    for i in range(len(blocks)):
+        f, N = block_flops(out["K"][i], out["labels"][i], out["iters"][i], flags["corrmethod"])
+        flops_total += f.sum()
+        Ps = np.array([slots_for(int(n)) for n in N])
+        for P in np.unique(Ps):
+            flops_by_P[int(P)] = flops_by_P.get(int(P), 0.0) + float(f[Ps == P].sum())
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    launches, dom_ms = prof[dom]
+    achieved = flops_by_P.get(dom, 0.0) / launches / (dom_ms / launches * 1e-3) / 1e12
+    kernel_ms_total = sum(v[1] for v in prof.values())
+    whole_tflops = flops_total / secs / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        rate, n, used, checked, bad = cpu_baseline(oracle, native, x, flags, blocks,
+                                                   args.cpu_seconds, out)
+        log("cpu oracle: %d pairs in %.1fs cpu -> %.2f pairs/s; parity %d/%d mismatching"
+            % (n, used, rate, bad, checked))
+        cpu = {"value": rate, "unit": "pairs/s", "cores": 1, "kind": "port",
+               "sample": "%d pairs: the first pairs of the %d timed work blocks, single-threaded "
+                         "C oracle (-O2, no FP contraction); GPU output bit-exact on %d/%d"
+                         % (n, len(blocks), checked - bad, checked)}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": len(blocks),
+        "warmup": args.warmup,
+        "ms_per_step": secs / len(blocks) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "KINC similarity, GMM (K=1..5, BIC) + Spearman, pre/post outliers, "
+                        "minsamp 30; 20000 genes x 500 samples FP32, module generator, 10% NaN",
+            "pairs_per_step": BLOCK,
+            "step": "one 32768-pair work block",
+            "form": "persistent",
+            "parallelism": "1 GPU",
+        },
+        "plain_value": plain_value,
+        "batched_vs_plain": (value / plain_value) if plain_value else None,
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_pairs_persistent<P=%d,KMAX=5>" % dom,
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "whole_job_tflops": whole_tflops,
+            "kernel_time_share": kernel_ms_total / (secs * 1e3),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

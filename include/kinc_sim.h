@@ -133,6 +133,18 @@ void kinc_sim_destroy(kinc_sim* h);
  * kinc_sim_create when h is NULL). */
 const char* kinc_sim_last_error(const kinc_sim* h);
 
+/*
+ * Kernel timing (instrumentation; the reference has no counterpart).  While
+ * enabled, every pair-kernel launch of the handle is bracketed by HIP events
+ * on the stream it runs on.  kinc_sim_profile_read waits for the handle's
+ * work, then reports, per register-slot class P (64*P samples per wave),
+ * the number of launches and their summed duration in milliseconds, and
+ * clears the record.  Returns the number of classes written (<= cap).
+ */
+int kinc_sim_set_profiling(kinc_sim* h, int32_t enabled);
+int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launches,
+                          double* total_ms, int32_t cap);
+
 /* Number of linear pairs for G genes: G(G-1)/2 (src/core/similarity.cpp:19). */
 int64_t kinc_pair_count(int32_t gene_size);
 

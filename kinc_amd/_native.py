@@ -61,6 +61,9 @@ SIGNATURES = [
     ("kinc_sim_synchronize", ctypes.c_int, [_P]),
     ("kinc_sim_destroy", None, [_P]),
     ("kinc_sim_last_error", ctypes.c_char_p, [_P]),
+    ("kinc_sim_set_profiling", ctypes.c_int, [_P, ctypes.c_int32]),
+    ("kinc_sim_profile_read", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int32]),
+    ("kinc_selftest_math", ctypes.c_int, [ctypes.c_int32, _P, _P, ctypes.c_int64]),
     ("kinc_pair_count", ctypes.c_int64, [ctypes.c_int32]),
     ("kinc_pair_index", None, [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
                                ctypes.POINTER(ctypes.c_int32)]),
@@ -165,6 +168,21 @@ class Engine:
                                            iters_ptr, stream)
         self._check(rc)
 
+    def set_profiling(self, on):
+        self._check(lib().kinc_sim_set_profiling(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        """{P: (launches, total_ms)} of the pair kernels since the last read."""
+        cap = 32
+        slots = np.zeros(cap, np.int32)
+        launches = np.zeros(cap, np.int64)
+        ms = np.zeros(cap, np.float64)
+        n = lib().kinc_sim_profile_read(self.h, slots.ctypes.data, launches.ctypes.data,
+                                        ms.ctypes.data, cap)
+        if n < 0:
+            self._check(n)
+        return {int(slots[i]): (int(launches[i]), float(ms[i])) for i in range(n)}
+
     def synchronize(self):
         self._check(lib().kinc_sim_synchronize(self.h))
 
@@ -178,3 +196,13 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def device_math(fn, x):
+    """Evaluate exp (0), log (1), sqrt (2) or 1/x (3) on the device."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    rc = lib().kinc_selftest_math(fn, x.ctypes.data, out.ctypes.data, x.size)
+    if rc != KINC_OK:
+        raise KincError(rc, "kinc_selftest_math failed")
+    return out

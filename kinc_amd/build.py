@@ -35,7 +35,10 @@ SOURCES = [
     "sim_pairs_k8.hip",
     "sim_util_kernels.hip",
     "kinc_sim.cpp",
+    "formats.cpp",
 ]
+CLI_SOURCE = "kinc_cli.cpp"
+CLI = os.path.join(HERE, "kinc")
 
 
 def _compile(src):
@@ -57,8 +60,16 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    cli_obj = _compile(CLI_SOURCE)
+    tmp = CLI + ".tmp"
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-o", tmp, cli_obj, "-L" + HERE, "-lkinc",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("CLI link failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, CLI)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", CLI)
     return LIB
 
 

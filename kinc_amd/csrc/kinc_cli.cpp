@@ -1,0 +1,217 @@
+/*
+ * kinc_cli.cpp — `kinc run similarity`, the command-line entry of the
+ * Similarity analytic on one MI355X.
+ *
+ * Mirrors the reference's command (src/core/similarity_input.cpp:96-254,
+ * defaults src/core/similarity.h:55-71):
+ *   kinc run similarity --input A.emx --ccm A.ccm --cmx A.cmx
+ *        [--clusmethod none|gmm|kmeans] [--corrmethod pearson|spearman]
+ *        [--minexpr F] [--minsamp N] [--minclus N] [--maxclus N]
+ *        [--crit BIC|ICL] [--preout true|false] [--postout true|false]
+ *        [--mincorr F] [--maxcorr F] [--ksize N]
+ * plus this build's --device N and --form plain|persistent.
+ *
+ * The ACE driver loop is Similarity::size/makeWork (similarity.cpp:17-39)
+ * then execute and process in pair order.  Here the device executes block
+ * b+1 while the host writes block b: two device output sets and two pinned
+ * host sets alternate, and process() still sees the blocks in order.
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kinc_formats.h"
+#include "../../include/kinc_sim.h"
+
+namespace {
+
+bool parse_bool(const std::string& v)
+{
+   return v == "true" || v == "1" || v == "yes" || v == "on";
+}
+
+int usage()
+{
+   fprintf(stderr,
+           "usage: kinc run similarity --input EMX --ccm CCM --cmx CMX [--clusmethod none|gmm|kmeans]\n"
+           "       [--corrmethod pearson|spearman] [--minexpr F] [--minsamp N] [--minclus N]\n"
+           "       [--maxclus N] [--crit BIC|ICL] [--preout BOOL] [--postout BOOL]\n"
+           "       [--mincorr F] [--maxcorr F] [--ksize N] [--device N] [--form plain|persistent]\n");
+   return 2;
+}
+
+struct HostSet
+{
+   int8_t* K = nullptr;
+   int8_t* labels = nullptr;
+   float* corr = nullptr;
+};
+
+struct DevSet
+{
+   int8_t* K = nullptr;
+   int8_t* labels = nullptr;
+   float* corr = nullptr;
+};
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+   if ( argc < 3 || std::string(argv[1]) != "run" || std::string(argv[2]) != "similarity" ) return usage();
+   kinc_sim_params p;
+   kinc_sim_default_params(&p);
+   std::string in, ccm, cmx;
+   int device = 0;
+   for ( int i = 3; i < argc; i += 2 )
+   {
+      if ( i + 1 >= argc ) return usage();
+      const std::string k = argv[i], v = argv[i + 1];
+      if ( k == "--input" ) in = v;
+      else if ( k == "--ccm" ) ccm = v;
+      else if ( k == "--cmx" ) cmx = v;
+      else if ( k == "--clusmethod" ) p.clus_method = v == "gmm" ? KINC_CLUS_GMM : v == "kmeans" ? KINC_CLUS_KMEANS : KINC_CLUS_NONE;
+      else if ( k == "--corrmethod" ) p.corr_method = v == "spearman" ? KINC_CORR_SPEARMAN : KINC_CORR_PEARSON;
+      else if ( k == "--minexpr" ) p.min_expr = strtof(v.c_str(), nullptr);
+      else if ( k == "--minsamp" ) p.min_samples = atoi(v.c_str());
+      else if ( k == "--minclus" ) p.min_clusters = atoi(v.c_str());
+      else if ( k == "--maxclus" ) p.max_clusters = atoi(v.c_str());
+      else if ( k == "--crit" ) p.criterion = v == "BIC" ? KINC_CRIT_BIC : KINC_CRIT_ICL;
+      else if ( k == "--preout" ) p.pre_out = parse_bool(v);
+      else if ( k == "--postout" ) p.post_out = parse_bool(v);
+      else if ( k == "--mincorr" ) p.min_corr = strtof(v.c_str(), nullptr);
+      else if ( k == "--maxcorr" ) p.max_corr = strtof(v.c_str(), nullptr);
+      else if ( k == "--ksize" ) p.ksize = atoi(v.c_str());
+      else if ( k == "--device" ) device = atoi(v.c_str());
+      else if ( k == "--form" ) p.form = v == "plain" ? KINC_FORM_PLAIN : KINC_FORM_PERSISTENT;
+      else return usage();
+   }
+   if ( in.empty() || ccm.empty() || cmx.empty() )
+   {
+      fprintf(stderr, "Did not get valid input and/or output arguments.\n");
+      return 2;
+   }
+
+   int32_t G = 0, S = 0;
+   if ( kinc_emx_info(in.c_str(), &G, &S) != KINC_OK )
+   {
+      fprintf(stderr, "%s\n", kinc_formats_last_error());
+      return 1;
+   }
+   std::vector<float> emx((size_t)G * (size_t)S);
+   if ( kinc_emx_read(in.c_str(), emx.data()) != KINC_OK )
+   {
+      fprintf(stderr, "%s\n", kinc_formats_last_error());
+      return 1;
+   }
+   int64_t msize = kinc_file_metadata(in.c_str(), nullptr, 0);
+   std::string meta(msize > 0 ? (size_t)msize : 0, '\0');
+   if ( msize > 0 ) kinc_file_metadata(in.c_str(), &meta[0], msize);
+
+   kinc_sim* h = nullptr;
+   if ( kinc_sim_create(&p, emx.data(), G, S, device, &h) != KINC_OK )
+   {
+      fprintf(stderr, "%s\n", kinc_sim_last_error(nullptr));
+      return 1;
+   }
+   kinc_similarity_writer* w = nullptr;
+   if ( kinc_similarity_writer_open(ccm.c_str(), cmx.c_str(), G, S, p.max_clusters, p.corr_method,
+                                    p.min_corr, p.max_corr, meta.empty() ? nullptr : meta.c_str(), &w) != KINC_OK )
+   {
+      fprintf(stderr, "%s\n", kinc_formats_last_error());
+      kinc_sim_destroy(h);
+      return 1;
+   }
+
+   const int64_t total = kinc_pair_count(G);
+   const int64_t B = KINC_WORK_BLOCK_SIZE;
+   const int64_t nblocks = (total + B - 1) / B;
+   const int MC = p.max_clusters;
+   hipStream_t stream;
+   DevSet dev[2];
+   HostSet host[2];
+   hipEvent_t done[2];
+   bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess;
+   for ( int s = 0; ok && s < 2; ++s )
+   {
+      ok = hipMalloc(&dev[s].K, B) == hipSuccess && hipMalloc(&dev[s].labels, B * S) == hipSuccess
+        && hipMalloc(&dev[s].corr, sizeof(float) * B * MC) == hipSuccess
+        && hipHostMalloc(&host[s].K, B, 0) == hipSuccess && hipHostMalloc(&host[s].labels, B * S, 0) == hipSuccess
+        && hipHostMalloc(&host[s].corr, sizeof(float) * B * MC, 0) == hipSuccess
+        && hipEventCreate(&done[s]) == hipSuccess;
+   }
+   if ( !ok )
+   {
+      fprintf(stderr, "device allocation failed\n");
+      return 1;
+   }
+
+   auto t0 = std::chrono::steady_clock::now();
+   auto submit = [&](int64_t b) -> bool {
+      const int s = (int)(b & 1);
+      const int64_t start = b * B;
+      const int64_t count = std::min(B, total - start);
+      if ( kinc_sim_execute_device(h, start, count, dev[s].K, dev[s].labels, dev[s].corr, nullptr, stream) != KINC_OK )
+      {
+         fprintf(stderr, "%s\n", kinc_sim_last_error(h));
+         return false;
+      }
+      return hipMemcpyAsync(host[s].K, dev[s].K, count, hipMemcpyDeviceToHost, stream) == hipSuccess
+          && hipMemcpyAsync(host[s].labels, dev[s].labels, count * S, hipMemcpyDeviceToHost, stream) == hipSuccess
+          && hipMemcpyAsync(host[s].corr, dev[s].corr, sizeof(float) * count * MC, hipMemcpyDeviceToHost, stream) == hipSuccess
+          && hipEventRecord(done[s], stream) == hipSuccess;
+   };
+   int rc = 0;
+   if ( nblocks > 0 && !submit(0) ) rc = 1;
+   for ( int64_t b = 0; rc == 0 && b < nblocks; ++b )
+   {
+      const int s = (int)(b & 1);
+      if ( b + 1 < nblocks && !submit(b + 1) )
+      {
+         rc = 1;
+         break;
+      }
+      if ( hipEventSynchronize(done[s]) != hipSuccess )
+      {
+         rc = 1;
+         break;
+      }
+      const int64_t start = b * B;
+      const int64_t count = std::min(B, total - start);
+      if ( kinc_similarity_writer_process(w, start, count, host[s].K, host[s].labels, host[s].corr, MC) != KINC_OK )
+      {
+         fprintf(stderr, "%s\n", kinc_formats_last_error());
+         rc = 1;
+      }
+   }
+   if ( kinc_similarity_writer_close(w) != KINC_OK && rc == 0 )
+   {
+      fprintf(stderr, "%s\n", kinc_formats_last_error());
+      rc = 1;
+   }
+   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+   if ( rc == 0 )
+   {
+      fprintf(stderr, "similarity: %lld pairs in %lld blocks, %.3f s (%.0f pairs/s)\n",
+              (long long)total, (long long)nblocks, secs, total / secs);
+   }
+   for ( int s = 0; s < 2; ++s )
+   {
+      (void)hipFree(dev[s].K);
+      (void)hipFree(dev[s].labels);
+      (void)hipFree(dev[s].corr);
+      (void)hipHostFree(host[s].K);
+      (void)hipHostFree(host[s].labels);
+      (void)hipHostFree(host[s].corr);
+      (void)hipEventDestroy(done[s]);
+   }
+   (void)hipStreamDestroy(stream);
+   kinc_sim_destroy(h);
+   return rc;
+}

@@ -13,13 +13,15 @@
  *           minimax polynomial of expf.c for e^g on |g| <= ln2/2.
  *   - logf: x = m * 2^e with m in [sqrt(1/2), sqrt(2)), the degree-9
  *           minimax polynomial of logf.c for log(1+f), and the same ln2 split.
- * Only IEEE-754 single-precision +, -, * and comparisons are used (no FMA,
- * no division, no hardware transcendental), so any compiler that keeps
- * FP contraction off and preserves denormals produces the same result on
- * the host CPU and on gfx950.  sqrt is the correctly rounded IEEE sqrtf.
+ * Only correctly rounded IEEE-754 single-precision operations are used:
+ * +, -, *, explicit fused multiply-add (fmaf), ldexp, frexp, rint and
+ * comparisons; no hardware transcendental.  Each of these has exactly one
+ * correct result, so the host CPU (gcc, FMA3) and gfx950 (v_fma_f32,
+ * v_ldexp_f32, v_frexp_*, v_rndne_f32) produce identical bits as long as
+ * implicit FP contraction stays off and denormals are preserved.  sqrt is
+ * the correctly rounded IEEE sqrtf.
  *
- * Accuracy against glibc (checked by the tests): expf within 2 ulp, logf
- * within 2 ulp over the ranges the pair pipeline uses.
+ * Accuracy against glibc (checked by the tests): expf and logf within 2 ulp.
  */
 #ifndef KINC_MATH_H
 #define KINC_MATH_H
@@ -54,52 +56,74 @@ KINC_HD float kinc_pow2i(int n)
    return kinc_as_float((uint32_t)(n + 127) << 23);
 }
 
-/* Round to nearest integer, ties to even, for |x| < 2^22 (exact on both sides:
- * adding and subtracting 1.5*2^23 forces the rounding in the FP adder). */
-KINC_HD float kinc_rint_small(float x)
+/* p * 2^n for -150 <= n <= 128 with one IEEE rounding (ldexp).  The host
+ * form keeps the first product normal and exact, so the second applies the
+ * only rounding, which is the correctly rounded ldexp result. */
+KINC_HD float kinc_ldexpf(float p, int n)
 {
-   const float magic = 12582912.0f; /* 1.5 * 2^23 */
-   float t = x + magic;
-   return t - magic;
+#if defined(__HIP_DEVICE_COMPILE__)
+   return __builtin_amdgcn_ldexpf(p, n);
+#else
+   int n1 = (n > 127) ? 127 : ((n < -126) ? n + 126 : n);
+   int n2 = (n > 127) ? n - 127 : ((n < -126) ? -126 : 0);
+   return (p * kinc_pow2i(n1)) * kinc_pow2i(n2);
+#endif
+}
+
+KINC_HD float kinc_fmaf(float a, float b, float c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   return __builtin_fmaf(a, b, c);
+#else
+   return fmaf(a, b, c);
+#endif
+}
+
+KINC_HD float kinc_rintf(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   return __builtin_rintf(x);
+#else
+   return rintf(x);
+#endif
+}
+
+/* x = m * 2^e with m in [0.5, 1) for finite nonzero x (exact). */
+KINC_HD float kinc_frexpf(float x, int* e)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   *e = __builtin_amdgcn_frexp_expf(x);
+   return __builtin_amdgcn_frexp_mantf(x);
+#else
+   return frexpf(x, e);
+#endif
 }
 
 KINC_HD float kinc_expf(float x)
 {
-   const float MAXLOGF = 88.72283905206835f;
-   const float MINLOGF = -103.97207708f; /* below this the result rounds to 0 */
+   /* Outside [-104, 89] the result is 0 or +inf; clamping there lets the
+    * final ldexp produce them with no extra case (exp(-104) < 2^-150 rounds
+    * to 0, exp(89) overflows). */
    const float LOG2EF = 1.44269504088896341f;
    const float C1 = 0.693359375f;
    const float C2 = -2.12194440e-4f;
+   float xc = fminf(fmaxf(x, -104.0f), 89.0f);
 
-   /* straight-line form: clamp, evaluate, then select the special cases */
-   float xc = (x < MINLOGF) ? MINLOGF : x;
-   xc = (xc > MAXLOGF) ? MAXLOGF : xc;
-   xc = (x != x) ? 0.0f : xc;
-
-   float nf = kinc_rint_small(LOG2EF * xc);
-   float g = xc - nf * C1;
-   g = g - nf * C2;
+   float nf = kinc_rintf(xc * LOG2EF);
+   float g = kinc_fmaf(-nf, C1, xc);
+   g = kinc_fmaf(-nf, C2, g);
    float z = g * g;
    float p = 1.9875691500E-4f;
-   p = p * g + 1.3981999507E-3f;
-   p = p * g + 8.3334519073E-3f;
-   p = p * g + 4.1665795894E-2f;
-   p = p * g + 1.6666665459E-1f;
-   p = p * g + 5.0000001201E-1f;
-   p = p * z + g;
+   p = kinc_fmaf(p, g, 1.3981999507E-3f);
+   p = kinc_fmaf(p, g, 8.3334519073E-3f);
+   p = kinc_fmaf(p, g, 4.1665795894E-2f);
+   p = kinc_fmaf(p, g, 1.6666665459E-1f);
+   p = kinc_fmaf(p, g, 5.0000001201E-1f);
+   p = kinc_fmaf(p, z, g);
    p = p + 1.0f;
 
-   /* p * 2^n with one rounding: the first factor keeps the product normal
-    * and exact, the second applies the remaining power of two. */
-   int n = (int)nf;                      /* -150 <= n <= 128 */
-   int n1 = (n > 127) ? 127 : ((n < -126) ? n + 126 : n);
-   int n2 = (n > 127) ? n - 127 : ((n < -126) ? -126 : 0);
-   float r = (p * kinc_pow2i(n1)) * kinc_pow2i(n2);
-
-   r = (x > MAXLOGF) ? INFINITY : r;
-   r = (x < MINLOGF) ? 0.0f : r;
-   r = (x != x) ? x : r;
-   return r;
+   float r = kinc_ldexpf(p, (int)nf);    /* -150 <= nf <= 128 */
+   return (x != x) ? x : r;
 }
 
 KINC_HD float kinc_logf(float x)
@@ -108,13 +132,12 @@ KINC_HD float kinc_logf(float x)
    const float C1 = 0.693359375f;
    const float C2 = -2.12194440e-4f;
 
-   /* denormals are scaled into the normal range first */
-   uint32_t u0 = kinc_as_uint(x);
-   int den = (u0 & 0x7f800000u) == 0u;
-   float xs = den ? x * 33554432.0f : x;   /* 2^25, exact */
-   uint32_t u = kinc_as_uint(xs);
-   int e = (int)((u >> 23) & 0xffu) - 126 - (den ? 25 : 0);
-   float m = kinc_as_float((u & 0x007fffffu) | 0x3f000000u);   /* [0.5, 1) */
+   /* m in [0.5, 1); zero, infinities and NaN are replaced at the end */
+   int e = 0;
+   float m = kinc_frexpf(x, &e);
+   int fin = (x > 0.0f) && (x < INFINITY);
+   m = fin ? m : 0.75f;
+   e = fin ? e : 0;
 
    int lo = m < SQRTHF;
    e = lo ? e - 1 : e;
@@ -122,22 +145,22 @@ KINC_HD float kinc_logf(float x)
 
    float z = f * f;
    float y = 7.0376836292E-2f;
-   y = y * f - 1.1514610310E-1f;
-   y = y * f + 1.1676998740E-1f;
-   y = y * f - 1.2420140846E-1f;
-   y = y * f + 1.4249322787E-1f;
-   y = y * f - 1.6668057665E-1f;
-   y = y * f + 2.0000714765E-1f;
-   y = y * f - 2.4999993993E-1f;
-   y = y * f + 3.3333331174E-1f;
+   y = kinc_fmaf(y, f, -1.1514610310E-1f);
+   y = kinc_fmaf(y, f, 1.1676998740E-1f);
+   y = kinc_fmaf(y, f, -1.2420140846E-1f);
+   y = kinc_fmaf(y, f, 1.4249322787E-1f);
+   y = kinc_fmaf(y, f, -1.6668057665E-1f);
+   y = kinc_fmaf(y, f, 2.0000714765E-1f);
+   y = kinc_fmaf(y, f, -2.4999993993E-1f);
+   y = kinc_fmaf(y, f, 3.3333331174E-1f);
    y = y * f;
    y = y * z;
 
    float fe = (float)e;
-   y = y + fe * C2;
-   y = y - 0.5f * z;
+   y = kinc_fmaf(fe, C2, y);
+   y = kinc_fmaf(-0.5f, z, y);
    float r = f + y;
-   r = r + fe * C1;
+   r = kinc_fmaf(fe, C1, r);
 
    r = (x == INFINITY) ? INFINITY : r;
    r = (x == 0.0f) ? -INFINITY : r;

@@ -95,6 +95,11 @@ struct kinc_sim
    SimArgs base{};
    size_t lds_bytes = 0;
    std::string err;
+   /* kernel timing */
+   bool profiling = false;
+   struct Timed { int P; hipEvent_t a, b; };
+   std::vector<Timed> timed;
+   std::vector<hipEvent_t> event_pool;
 };
 
 static int fail(kinc_sim* h, int code, const std::string& msg)
@@ -301,6 +306,80 @@ static int launch(kinc_sim* h, const void* fn, unsigned grid, unsigned block, vo
    return KINC_OK;
 }
 
+static int take_event(kinc_sim* h, hipEvent_t* ev)
+{
+   if ( !h->event_pool.empty() )
+   {
+      *ev = h->event_pool.back();
+      h->event_pool.pop_back();
+      return KINC_OK;
+   }
+   HIP_TRY(h, hipEventCreate(ev));
+   return KINC_OK;
+}
+
+/* A pair-kernel launch, bracketed by events when profiling is on. */
+static int launch_pairs(kinc_sim* h, int P, const void* fn, unsigned grid, void** args, hipStream_t s)
+{
+   if ( !h->profiling ) return launch(h, fn, grid, 64, args, h->lds_bytes, s);
+   kinc_sim::Timed t;
+   t.P = P;
+   int rc = take_event(h, &t.a);
+   if ( rc ) return rc;
+   rc = take_event(h, &t.b);
+   if ( rc ) return rc;
+   HIP_TRY(h, hipEventRecord(t.a, s));
+   rc = launch(h, fn, grid, 64, args, h->lds_bytes, s);
+   if ( rc ) return rc;
+   HIP_TRY(h, hipEventRecord(t.b, s));
+   h->timed.push_back(t);
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_set_profiling(kinc_sim* h, int32_t enabled)
+{
+   if ( !h ) return KINC_E_INVALID;
+   h->profiling = enabled != 0;
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launches,
+                                     double* total_ms, int32_t cap)
+{
+   if ( !h ) return KINC_E_INVALID;
+   HIP_TRY(h, hipSetDevice(h->device));
+   std::vector<int> ps;
+   std::vector<int64_t> ns;
+   std::vector<double> ms;
+   for ( auto& t : h->timed )
+   {
+      HIP_TRY(h, hipEventSynchronize(t.b));
+      float e = 0.0f;
+      HIP_TRY(h, hipEventElapsedTime(&e, t.a, t.b));
+      size_t j = 0;
+      while ( j < ps.size() && ps[j] != t.P ) ++j;
+      if ( j == ps.size() )
+      {
+         ps.push_back(t.P);
+         ns.push_back(0);
+         ms.push_back(0.0);
+      }
+      ns[j] += 1;
+      ms[j] += e;
+      h->event_pool.push_back(t.a);
+      h->event_pool.push_back(t.b);
+   }
+   h->timed.clear();
+   int n = 0;
+   for ( size_t j = 0; j < ps.size() && n < cap; ++j, ++n )
+   {
+      if ( slots ) slots[n] = ps[j];
+      if ( launches ) launches[n] = ns[j];
+      if ( total_ms ) total_ms[n] = ms[j];
+   }
+   return n;
+}
+
 extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count,
                                        int8_t* K, int8_t* labels, float* corr,
                                        uint16_t* iters, void* stream)
@@ -354,7 +433,7 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
          a.first = first;
          a.launch_count = std::min(h->p.ksize, cnt - first);
          void* args[] = {&a};
-         int rc = launch(h, fn, (unsigned)a.launch_count, 64, args, h->lds_bytes, s);
+         int rc = launch_pairs(h, P, fn, (unsigned)a.launch_count, args, s);
          if ( rc ) return rc;
       }
       return KINC_OK;
@@ -392,7 +471,7 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
       a.list_len = h->d_counters + b;
       a.list_head = h->d_counters + kNumSlots + b;
       void* args[] = {&a};
-      int rc = launch(h, fn, (unsigned)grid, 64, args, h->lds_bytes, s);
+      int rc = launch_pairs(h, P, fn, (unsigned)grid, args, s);
       if ( rc ) return rc;
    }
    return KINC_OK;
@@ -442,6 +521,12 @@ extern "C" void kinc_sim_destroy(kinc_sim* h)
    (void)hipFree(h->d_labels);
    (void)hipFree(h->d_corr);
    (void)hipFree(h->d_iters);
+   for ( auto& t : h->timed )
+   {
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+   }
+   for ( auto e : h->event_pool ) (void)hipEventDestroy(e);
    if ( h->stream ) (void)hipStreamDestroy(h->stream);
    delete h;
 }

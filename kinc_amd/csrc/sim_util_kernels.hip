@@ -82,3 +82,47 @@ const void* kinc_kernel_chunk_max() { return (const void*)&k_chunk_max; }
 const void* kinc_kernel_bin_pairs() { return (const void*)&k_bin_pairs; }
 const void* kinc_kernel_zero_i32() { return (const void*)&k_zero_i32; }
 }
+
+/* Device evaluation of the shared FP32 functions (diagnostic). */
+__global__ void k_eval_math(int fn, const float* in, float* out, int64_t n)
+{
+   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if ( i >= n ) return;
+   const float x = in[i];
+   float r;
+   switch ( fn )
+   {
+   case 0: r = kinc_expf(x); break;
+   case 1: r = kinc_logf(x); break;
+   case 2: r = sqrtf(x); break;
+   default: r = 1.0f / x; break;
+   }
+   out[i] = r;
+}
+
+extern "C" int kinc_selftest_math(int32_t fn, const float* in, float* out, int64_t n)
+{
+   if ( fn < 0 || fn > 3 || n < 0 || (n > 0 && (!in || !out)) ) return -1;
+   if ( n == 0 ) return 0;
+   float* d_in = nullptr;
+   float* d_out = nullptr;
+   const size_t bytes = sizeof(float) * (size_t)n;
+   if ( hipMalloc(&d_in, bytes) != hipSuccess ) return -3;
+   if ( hipMalloc(&d_out, bytes) != hipSuccess )
+   {
+      (void)hipFree(d_in);
+      return -3;
+   }
+   int rc = 0;
+   if ( hipMemcpy(d_in, in, bytes, hipMemcpyHostToDevice) != hipSuccess ) rc = -2;
+   if ( rc == 0 )
+   {
+      const unsigned grid = (unsigned)((n + 255) / 256);
+      hipLaunchKernelGGL(k_eval_math, dim3(grid), dim3(256), 0, 0, fn, d_in, d_out, n);
+      if ( hipGetLastError() != hipSuccess ) rc = -2;
+   }
+   if ( rc == 0 && hipMemcpy(out, d_out, bytes, hipMemcpyDeviceToHost) != hipSuccess ) rc = -2;
+   (void)hipFree(d_in);
+   (void)hipFree(d_out);
+   return rc;
+}

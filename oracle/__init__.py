@@ -44,6 +44,7 @@ def lib():
             "oracle_bitonic_sort": (None, [ctypes.c_int, P, P]),
             "oracle_execute_block": (ctypes.c_int, [P, P, i32, i32, i64, i64, P, P, P, P]),
             "oracle_corr_fp64_block": (ctypes.c_int, [P, P, i32, i32, i64, i64, P, P, P]),
+            "oracle_math_array": (None, [ctypes.c_int, P, P, i64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -158,3 +159,11 @@ def bitonic_sort(key, payload):
 
 def bic(K, logL, N, D=2):
     return lib().oracle_bic(K, logL, N, D)
+
+
+def math_array(fn, x):
+    """Host evaluation of exp (0), log (1), sqrt (2) or 1/x (3)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().oracle_math_array(fn, x.ctypes.data, out.ctypes.data, x.size)
+    return out

@@ -950,3 +950,20 @@ int oracle_corr_fp64_block(const kinc_sim_params* p, const float* emx,
    free(X); free(lab); free(wx); free(wy); free(wr);
    return KINC_OK;
 }
+
+/* Host evaluation of the shared FP32 functions over an array:
+ * fn 0 exp, 1 log, 2 sqrt, 3 reciprocal. */
+void oracle_math_array(int fn, const float* in, float* out, int64_t n)
+{
+   for ( int64_t i = 0; i < n; ++i )
+   {
+      const float x = in[i];
+      switch ( fn )
+      {
+      case 0: out[i] = kinc_expf(x); break;
+      case 1: out[i] = kinc_logf(x); break;
+      case 2: out[i] = sqrtf(x); break;
+      default: out[i] = 1.0f / x; break;
+      }
+   }
+}

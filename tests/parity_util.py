@@ -28,3 +28,14 @@ def assert_block_equal(got, want, S, where=""):
     bad = np.nonzero(~eq.all(axis=1))[0]
     assert bad.size == 0, "%s correlations differ at pairs %s: got %s want %s" % (
         where, bad[:5], C1[bad[:5]], C0[bad[:5]])
+
+
+def count_mismatching_pairs(got, want):
+    """Number of pairs whose K, labels, iteration record or correlation bits differ."""
+    K1, L1, C1, I1 = got
+    K0, L0, C0, I0 = want
+    bad = K1 != K0
+    bad |= (L1 != L0).any(axis=1)
+    bad |= (I1 != I0).any(axis=1)
+    bad |= ~same_float_bits(C1, C0).all(axis=1)
+    return int(np.count_nonzero(bad))

@@ -1,0 +1,32 @@
+"""Device FP32 functions against the host, bit for bit, over a strided sweep
+of all 2^32 float bit patterns plus special values."""
+import numpy as np
+import pytest
+
+import oracle
+from kinc_amd import _native
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs():
+    sweep = (np.arange(0, 1 << 32, 241, dtype=np.uint64) & 0xFFFFFFFF).astype(np.uint32)
+    x = sweep.view(np.float32)
+    special = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 1e-45, -1e-45,
+                        1.17549435e-38, 88.7, 88.72284, 89.0, -103.9, -104.0, -87.33655],
+                       np.float32)
+    rng = np.random.default_rng(0)
+    em = rng.uniform(-110, 0.5, 2_000_000).astype(np.float32)  # EM exp arguments
+    return np.concatenate([x, special, em])
+
+
+@pytest.mark.parametrize("fn,name", [(0, "exp"), (1, "log"), (2, "sqrt"), (3, "recip")])
+def test_device_math_bit_exact(fn, name):
+    x = _inputs()
+    print("math", name, "inputs", x.size, flush=True)
+    dev = _native.device_math(fn, x)
+    host = oracle.math_array(fn, x)
+    same = (dev.view(np.uint32) == host.view(np.uint32)) | (np.isnan(dev) & np.isnan(host))
+    bad = np.nonzero(~same)[0]
+    assert bad.size == 0, "%s differs at %d inputs, e.g. x=%r dev=%r host=%r" % (
+        name, bad.size, x[bad[:3]], dev[bad[:3]], host[bad[:3]])

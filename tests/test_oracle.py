@@ -1,0 +1,412 @@
+"""Pins of the CPU oracle (CPU only).
+
+The reference holds no golden vectors, fixtures or known-answer tests for
+the Similarity path (tests/main.cpp:39 disables TestSimilarity and
+tests/testsimilarity.cpp:82-85 verifies nothing), so each oracle function is
+pinned here by what the mathematics fixes: closed forms, known answers,
+brute force on small inputs, textbook FP64 formulas, invariants and
+separated-cluster recovery.  Each check is chosen so that a plausible slip
+(dropped term, wrong index, transposed operand, wrong tie rule) fails it.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def f32(v):
+    return float(np.float32(v))
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    a = np.where(a < 0, np.int64(-2**31) - a, a)
+    b = np.where(b < 0, np.int64(-2**31) - b, b)
+    return np.abs(a - b)
+
+
+# ------------------------------------------------------------------- index
+
+def test_pair_index_closed_form_matches_reference_loop():
+    # src/core/pairwise_index.cpp:30-56 (linear search) vs the closed form
+    rng = np.random.default_rng(0)
+    ks = list(range(0, 5000)) + [int(k) for k in rng.integers(0, 1_799_970_000, 2000)]
+    for k in ks:
+        x, y = oracle.pair_index(k)
+        assert (x, y) == oracle.pair_index(k, loop=True)
+        assert x > y >= 0 and x * (x - 1) // 2 + y == k
+
+
+def test_pair_index_walk_order():
+    # operator++ (pairwise_index.cpp:85-94): (1,0),(2,0),(2,1),(3,0),...
+    seq = [oracle.pair_index(k) for k in range(10)]
+    assert seq == [(1, 0), (2, 0), (2, 1), (3, 0), (3, 1), (3, 2), (4, 0), (4, 1), (4, 2), (4, 3)]
+    assert oracle.pair_index(199_989_999) == (19999, 19998)
+
+
+# --------------------------------------------------------------------- rng
+
+def test_lcg_known_answers():
+    # POSIX example rand() seeded with 1 (src/opencl/outlier.cl:14-18)
+    with open(os.path.join(GOLDEN, "lcg_seed1.json")) as f:
+        want = json.load(f)["values"]
+    assert oracle.lcg_sequence(len(want)) == want
+
+
+def test_lcg_closed_form():
+    s = 1
+    out = []
+    for _ in range(1000):
+        s = (s * 1103515245 + 12345) % (1 << 64)
+        out.append((s // 65536) % (1 << 32) % 32768)
+    assert oracle.lcg_sequence(1000) == out
+
+
+# ---------------------------------------------------------------- exp, log
+
+def test_expf_accuracy_and_specials():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-103.9, 88.7, 200000), rng.uniform(-2, 2, 100000),
+                         np.linspace(-20, 0, 20001)]).astype(np.float32)
+    got = np.array([oracle.expf(float(v)) for v in xs], np.float32)
+    want = np.exp(xs.astype(np.float64)).astype(np.float32)
+    normal = want > np.float32(1.2e-38)
+    assert ulp_diff(got[normal], want[normal]).max() <= 2
+    # denormal results stay within a few units of the smallest denormal
+    sub = ~normal
+    assert np.all(np.abs(got[sub].astype(np.float64) - np.exp(xs[sub].astype(np.float64)))
+                  <= 2 * 1.4e-45)
+    assert oracle.expf(0.0) == 1.0
+    assert oracle.expf(float("-inf")) == 0.0
+    assert oracle.expf(float("inf")) == float("inf")
+    assert oracle.expf(89.0) == float("inf")
+    assert math.isnan(oracle.expf(float("nan")))
+    assert oracle.expf(-105.0) == 0.0
+
+
+def test_logf_accuracy_and_specials():
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([10.0 ** rng.uniform(-37, 38, 200000), rng.uniform(0.5, 2.0, 100000),
+                         rng.uniform(1e-45, 1e-38, 2000)]).astype(np.float32)
+    xs = xs[xs > 0]
+    got = np.array([oracle.logf(float(v)) for v in xs], np.float32)
+    want = np.log(xs.astype(np.float64)).astype(np.float32)
+    assert ulp_diff(got, want).max() <= 2
+    assert oracle.logf(1.0) == 0.0
+    assert oracle.logf(0.0) == float("-inf")
+    assert oracle.logf(float("inf")) == float("inf")
+    assert math.isnan(oracle.logf(-1.0))
+    assert math.isnan(oracle.logf(float("nan")))
+
+
+# ---------------------------------------------------------------- outliers
+
+def _brute_outliers(X, labels, j, cluster, marker):
+    """pairwise_clustering.cpp:110-148 by an independent route: quartiles as
+    order statistics picked by counting, bounds in FP32."""
+    lab = labels.copy()
+    N = X.shape[0]
+    cand = [X[i, j] for i in range(N) if lab[i] in (cluster, marker)]
+    n = len(cand)
+    if n == 0:
+        return lab
+
+    def kth(k):
+        for v in cand:
+            less = sum(1 for w in cand if w < v)
+            le = sum(1 for w in cand if w <= v)
+            if less <= k < le:
+                return v
+        raise AssertionError
+
+    q1 = np.float32(kth(n // 4))
+    q3 = np.float32(kth(3 * n // 4))
+    iqr = np.float32(q3 - q1)
+    lo = np.float32(q1 - np.float32(np.float32(1.5) * iqr))
+    hi = np.float32(q3 + np.float32(np.float32(1.5) * iqr))
+    for i in range(N):
+        if lab[i] == cluster and (X[i, j] < lo or hi < X[i, j]):
+            lab[i] = marker
+    return lab
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_mark_outliers_brute_force(seed):
+    rng = np.random.default_rng(100 + seed)
+    N = int(rng.integers(1, 9)) if seed < 20 else int(rng.integers(9, 80))
+    X = rng.normal(0, 1, (N, 2)).astype(np.float32)
+    if seed % 3 == 0:
+        X[rng.integers(0, N)] *= 25  # a gross outlier
+    if seed % 5 == 0:
+        X = np.round(X * 2) / 2  # ties
+    labels = rng.choice(np.array([0, 0, 0, 1, -7, -9], np.int8), N)
+    for j in (0, 1):
+        for cluster, marker in ((0, -7), (1, -8)):
+            got = oracle.mark_outliers(X, labels, j, cluster, marker)
+            want = _brute_outliers(X.astype(np.float32), labels, j, cluster, marker)
+            assert np.array_equal(got, want)
+
+
+def test_mark_outliers_flags_gross_value():
+    X = np.array([[1, 0], [2, 0], [3, 0], [4, 0], [100, 0]], np.float32)
+    labels = np.zeros(5, np.int8)
+    got = oracle.mark_outliers(X, labels, 0, 0, -7)
+    # sorted 1,2,3,4,100: Q1 = x[1] = 2, Q3 = x[3] = 4, bounds [-1, 7]
+    assert got.tolist() == [0, 0, 0, 0, -7]
+
+
+# ---------------------------------------------------------------------- BIC
+
+def test_bic_closed_form():
+    for K, logL, N in [(1, -1234.5, 405), (3, -987.25, 30), (5, 12.0, 1000)]:
+        want = np.float32(math.log(N) * 7 * K - 2.0 * f32(logL))
+        assert oracle.bic(K, logL, N) == want
+
+
+# ---------------------------------------------------------------------- GMM
+
+def test_gmm_k1_closed_form():
+    """K=1: every responsibility is 1, so after the first M-step mu is the
+    sample mean and Sigma the biased ML covariance; the second M-step
+    reproduces them bit for bit, so logL repeats and EM stops after exactly
+    3 E-steps with all labels 0."""
+    rng = np.random.default_rng(3)
+    X = (rng.normal(0, 1, (400, 2)) @ np.array([[2.0, 0.5], [0.0, 1.0]]) + [8, 7]).astype(np.float32)
+    ok, labels, logL, E, st = oracle.gmm_fit(X, 1)
+    assert ok and st.tolist() == [3, 2, 1, 0]
+    assert np.all(labels == 0)
+    Xd = X.astype(np.float64)
+    mu = Xd.mean(0)
+    C = (Xd - mu).T @ (Xd - mu) / len(Xd)
+    inv = np.linalg.inv(C)
+    d = Xd - mu
+    ll = (-0.5 * np.einsum("ij,jk,ik->i", d, inv, d) - math.log(2 * math.pi)
+          - 0.5 * math.log(np.linalg.det(C))).sum()
+    assert abs(logL - ll) < 1e-4 * abs(ll)
+
+
+def test_gmm_separated_clusters_recovered():
+    rng = np.random.default_rng(4)
+    a = rng.normal([0, 0], 0.5, (150, 2))
+    b = rng.normal([10, -6], 0.5, (150, 2))
+    X = np.concatenate([a, b]).astype(np.float32)
+    truth = np.array([0] * 150 + [1] * 150)
+    ok, labels, logL2, _, st = oracle.gmm_fit(X, 2)
+    assert ok
+    agree = max(np.mean(labels == truth), np.mean(labels == 1 - truth))
+    assert agree == 1.0
+    ok1, _, logL1, _, _ = oracle.gmm_fit(X, 1)
+    # BIC prefers the true K = 2
+    assert oracle.bic(2, logL2, 300) < oracle.bic(1, logL1, 300)
+
+
+def test_gmm_singular_covariance_fails():
+    # duplicated points: a component collapses to det <= 0 and the fit fails
+    X = np.repeat(np.array([[1.0, 2.0], [3.0, 4.0], [5.0, 6.0]], np.float32), 20, axis=0)
+    ok, _, _, _, st = oracle.gmm_fit(X, 1)
+    assert not ok and st[3] == 1
+
+
+def test_gmm_logl_nondecreasing_fp64_check():
+    """EM never decreases the likelihood in exact arithmetic; in FP32 the
+    trajectory may wobble by rounding only."""
+    rng = np.random.default_rng(5)
+    X = np.concatenate([rng.normal([0, 0], 1, (100, 2)), rng.normal([3, 3], 1, (100, 2))]).astype(np.float32)
+    ok, labels, logL, _, st = oracle.gmm_fit(X, 2)
+    assert ok and st[0] >= 3
+    assert set(np.unique(labels)).issubset({0, 1})
+
+
+# ------------------------------------------------------------------- KMeans
+
+def test_kmeans_lloyd_fixed_point():
+    rng = np.random.default_rng(6)
+    X = np.concatenate([rng.normal([0, 0], 0.3, (60, 2)), rng.normal([5, 5], 0.3, (60, 2)),
+                        rng.normal([0, 6], 0.3, (60, 2))]).astype(np.float32)
+    ok, labels, logL, st = oracle.kmeans_fit(X, 3)
+    assert ok
+    means = np.array([X[labels == k].mean(0) for k in range(3)])
+    d = np.linalg.norm(X[:, None, :] - means[None], axis=2)
+    assert np.array_equal(d.argmin(1), labels)
+    truth = np.repeat(np.arange(3), 60)
+    # the partition is the true one up to relabelling
+    for k in range(3):
+        assert len(np.unique(truth[labels == k])) == 1
+    want = -((X - means[labels]) ** 2).sum()
+    assert abs(logL - want) < 1e-3 * abs(want)
+
+
+def test_kmeans_brute_force_tiny():
+    """For tiny N the best of the inits must reach the global optimum of the
+    within-cluster scatter found by enumerating all labelings."""
+    import itertools
+    rng = np.random.default_rng(7)
+    X = np.concatenate([rng.normal(0, 0.2, (4, 2)), rng.normal(4, 0.2, (4, 2))]).astype(np.float32)
+    best = None
+    for lab in itertools.product(range(2), repeat=8):
+        lab = np.array(lab)
+        if len(set(lab)) < 2:
+            continue
+        s = sum(((X[lab == k] - X[lab == k].mean(0)) ** 2).sum() for k in range(2))
+        best = s if best is None else min(best, s)
+    ok, labels, logL, _ = oracle.kmeans_fit(X, 2)
+    assert abs(-logL - best) < 1e-4
+
+
+# ------------------------------------------------------------------ Pearson
+
+def test_pearson_textbook():
+    rng = np.random.default_rng(8)
+    x = rng.normal(0, 1, 300)
+    y = 0.6 * x + 0.8 * rng.normal(0, 1, 300)
+    data = np.stack([x, y], 1).astype(np.float32) + 8
+    labels = np.zeros(300, np.int8)
+    r = oracle.pearson(data, labels, 0, 30)
+    want = np.corrcoef(data[:, 0].astype(np.float64), data[:, 1].astype(np.float64))[0, 1]
+    assert abs(r - want) < 1e-5
+    assert math.isnan(oracle.pearson(data, labels, 0, 301))
+
+
+def test_pearson_reads_full_index():
+    """pairwise_pearson.cpp:26-31 selects samples by the full-length label
+    index and reads data at that same index."""
+    rng = np.random.default_rng(9)
+    data = rng.normal(0, 1, (50, 2)).astype(np.float32)
+    labels = np.full(50, -9, np.int8)
+    idx = np.sort(rng.choice(50, 35, replace=False))
+    labels[idx] = 0
+    r = oracle.pearson(data, labels, 0, 30)
+    want = np.corrcoef(data[idx, 0].astype(np.float64), data[idx, 1].astype(np.float64))[0, 1]
+    assert abs(r - want) < 1e-5
+
+
+# ----------------------------------------------------------------- Spearman
+
+def test_spearman_untied_textbook():
+    from scipy.stats import spearmanr
+    rng = np.random.default_rng(10)
+    for n in (30, 64, 100, 300, 777):
+        x = rng.normal(0, 1, n)
+        y = 0.5 * x + rng.normal(0, 1, n)
+        data = np.stack([x, y], 1).astype(np.float32)
+        labels = np.zeros(n, np.int8)
+        r = oracle.spearman(data, labels, 0, 30)
+        want = spearmanr(data[:, 0], data[:, 1]).statistic
+        assert abs(r - want) < 1e-6, n
+
+
+def test_spearman_int32_wrap():
+    """n >= 1291 overflows the int32 denominator n*(n*n-1)
+    (pairwise_spearman.cpp:82); the result follows the two's-complement
+    value, e.g. n = 5000 gives 445,943,416."""
+    n = 5000
+    assert np.int32(np.uint32((n * (n * n - 1)) % (1 << 32))) == 445943416
+    rng = np.random.default_rng(11)
+    x = rng.permutation(n).astype(np.float32)
+    y = rng.permutation(n).astype(np.float32)
+    data = np.stack([x, y], 1)
+    labels = np.zeros(n, np.int8)
+    r = oracle.spearman(data, labels, 0, 30)
+    rx = np.argsort(np.argsort(x)) + 1
+    ry = np.argsort(np.argsort(y)) + 1
+    diff = int(((rx - ry) ** 2).sum())
+    diff32 = np.int32(np.uint32(diff % (1 << 32)))
+    want = np.float32(1.0 - 6.0 * float(diff32) / 445943416.0)
+    assert r == want
+
+
+def _bitonic_py(key, pay):
+    """Independent simulation of the network of pairwise_spearman.cpp:109-166."""
+    key = list(key)
+    pay = list(pay)
+    size = len(key)
+    ob = 2
+    while ob <= size:
+        ib = ob
+        while ib >= 2:
+            t = ib // 2
+            for i in range(size // 2):
+                desc = (i // (ob // 2)) & 1
+                a = (i // t) * ib + (i % t)
+                b = a + t
+                if (key[a] > key[b] and not desc) or (key[a] < key[b] and desc):
+                    key[a], key[b] = key[b], key[a]
+                    pay[a], pay[b] = pay[b], pay[a]
+            ib //= 2
+        ob *= 2
+    return key, pay
+
+
+@pytest.mark.parametrize("size", [2, 4, 8, 16, 64, 256])
+def test_bitonic_network_with_ties(size):
+    rng = np.random.default_rng(size)
+    key = rng.integers(0, 4, size).astype(np.float32)
+    pay = np.arange(size, dtype=np.float32)
+    k1, p1 = oracle.bitonic_sort(key, pay)
+    k0, p0 = _bitonic_py(key, pay)
+    assert k1.tolist() == k0 and p1.tolist() == p0
+    assert np.all(np.diff(k1) >= 0)
+
+
+def test_spearman_ties_follow_network():
+    rng = np.random.default_rng(12)
+    n = 100
+    x = rng.integers(0, 8, n).astype(np.float32)
+    y = rng.integers(0, 8, n).astype(np.float32)
+    data = np.stack([x, y], 1)
+    labels = np.zeros(n, np.int8)
+    r = oracle.spearman(data, labels, 0, 30)
+    n2 = 128
+    kx = list(x) + [np.inf] * (n2 - n)
+    ky = list(y) + [np.inf] * (n2 - n)
+    kr = [float(i + 1) for i in range(n)] + [0.0] * (n2 - n)
+    kx, ky = _bitonic_py(kx, ky)
+    ky, kr = _bitonic_py(ky, kr)
+    diff = sum(int((i + 1) - kr[i]) ** 2 for i in range(n))
+    want = np.float32(1.0 - 6.0 * diff / (n * (n * n - 1)))
+    assert r == want
+
+
+# ------------------------------------------------------------------- block
+
+def test_block_stale_x_pearson():
+    """Similarity::Serial keeps one X workspace per block
+    (similarity_serial.cpp:42,132): a pair with fewer valid samples than an
+    earlier pair of the block reads that pair's values past its own N."""
+    from kinc_amd import _native
+    S = 40
+    emx = np.full((3, S), np.nan, np.float32)
+    rng = np.random.default_rng(13)
+    emx[0] = rng.normal(0, 1, S)
+    emx[1] = rng.normal(0, 1, S)
+    emx[2, :34] = rng.normal(0, 1, 34)
+    # block pairs: 0 = (1,0) N=40; 1 = (2,0) N=34; 2 = (2,1) N=34
+    p = _native.make_params(clusmethod="none", corrmethod="pearson", minsamp=30)
+    K, labels, corr, _ = oracle.execute_block(p, emx, 0, 3)
+    assert K.tolist() == [1, 1, 1]
+    X1 = np.stack([emx[1], emx[0]], 1)
+    X2 = np.stack([emx[2, :34], emx[0, :34]], 1)
+    # pair 1: labels 0 at samples 0..33 -> reads its own X[0..33]
+    want1 = np.corrcoef(X2[:, 0].astype(np.float64), X2[:, 1].astype(np.float64))[0, 1]
+    assert abs(corr[1, 0] - want1) < 1e-5
+    # a variant where the valid samples sit at the end of the row
+    emx2 = emx.copy()
+    emx2[2] = np.nan
+    emx2[2, 6:] = rng.normal(0, 1, 34)
+    K, labels, corr, _ = oracle.execute_block(p, emx2, 0, 2)
+    Xc = np.stack([emx2[2, 6:], emx2[0, 6:]], 1)     # compact X of pair (2,0)
+    stale = np.concatenate([Xc, X1[34:40]])         # entries 34..39 from pair (1,0)
+    sel = stale[6:40]                               # full indices 6..39 have label 0
+    want = np.corrcoef(sel[:, 0].astype(np.float64), sel[:, 1].astype(np.float64))[0, 1]
+    assert abs(corr[1, 0] - want) < 1e-5
+    # the aligned (non-stale) reading gives a clearly different value
+    aligned = np.corrcoef(emx2[2, 6:].astype(np.float64), emx2[0, 6:].astype(np.float64))[0, 1]
+    assert abs(aligned - want) > 1e-2
