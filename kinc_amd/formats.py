@@ -16,29 +16,36 @@ MAGIC = b"KINCAMD1"
 TYPES = {1: "emx", 2: "ccm", 3: "cmx"}
 
 
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "kinc_emx_write": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_char_p]),
+    "kinc_emx_info": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),
+    "kinc_emx_read": (ctypes.c_int, [ctypes.c_char_p, _P]),
+    "kinc_file_metadata": (ctypes.c_int64, [ctypes.c_char_p, _P, ctypes.c_int64]),
+    "kinc_similarity_writer_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p,
+                                                   ctypes.c_int32, ctypes.c_int32,
+                                                   ctypes.c_int32, ctypes.c_int32,
+                                                   ctypes.c_float, ctypes.c_float,
+                                                   ctypes.c_char_p, _P]),
+    "kinc_similarity_writer_process": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P, _P,
+                                                      _P, ctypes.c_int32]),
+    "kinc_similarity_writer_close": (ctypes.c_int, [_P]),
+    "kinc_formats_last_error": (ctypes.c_char_p, []),
+}
+_bound = False
+
+
 def _bind():
+    global _bound
     l = _native.lib()
-    P = ctypes.c_void_p
-    sig = {
-        "kinc_emx_write": (ctypes.c_int, [ctypes.c_char_p, P, ctypes.c_int32, ctypes.c_int32,
-                                          ctypes.c_char_p]),
-        "kinc_emx_info": (ctypes.c_int, [ctypes.c_char_p, P, P]),
-        "kinc_emx_read": (ctypes.c_int, [ctypes.c_char_p, P]),
-        "kinc_file_metadata": (ctypes.c_int64, [ctypes.c_char_p, P, ctypes.c_int64]),
-        "kinc_similarity_writer_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p,
-                                                       ctypes.c_int32, ctypes.c_int32,
-                                                       ctypes.c_int32, ctypes.c_int32,
-                                                       ctypes.c_float, ctypes.c_float,
-                                                       ctypes.c_char_p, P]),
-        "kinc_similarity_writer_process": (ctypes.c_int, [P, ctypes.c_int64, ctypes.c_int64, P, P,
-                                                          P, ctypes.c_int32]),
-        "kinc_similarity_writer_close": (ctypes.c_int, [P]),
-        "kinc_formats_last_error": (ctypes.c_char_p, []),
-    }
-    for name, (res, args) in sig.items():
+    if _bound:
+        return l
+    for name, (res, args) in SIGNATURES.items():
         fn = getattr(l, name)
         fn.restype = res
         fn.argtypes = args
+    _bound = True
     return l
 
 
