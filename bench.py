@@ -159,7 +159,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}))
+        return
 
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; the pair space would shard "
@@ -252,7 +266,7 @@ def main():
         "batched_vs_plain": (value / plain_value) if plain_value else None,
         "roofline": {
             "bound": "valu",
-            "kernel": "k_pairs_persistent<P=%d,KMAX=5>" % dom,
+            "kernel": "k_pairs<P=%d,KMAX=5> (persistent form)" % dom,
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",

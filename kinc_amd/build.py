@@ -31,19 +31,20 @@ HIP_FLAGS = [
 ]
 
 SOURCES = [
-    "sim_pairs_k5.hip",
-    "sim_pairs_k8.hip",
     "sim_util_kernels.hip",
     "kinc_sim.cpp",
     "formats.cpp",
 ]
+# pair-kernel instances: (KMAX, P) for every register-slot count P
+SLOTS = (1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16)
+INSTANCES = [(k, p) for k in (5, 8) for p in SLOTS]
 CLI_SOURCE = "kinc_cli.cpp"
 CLI = os.path.join(HERE, "kinc")
 
 
-def _compile(src):
-    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", out]
+def _compile(src, defines=(), tag=None):
+    out = os.path.join(OBJ, (tag or os.path.splitext(src)[0]) + ".o")
+    cmd = [HIPCC] + HIP_FLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
@@ -52,8 +53,12 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    jobs = [(src, (), None) for src in SOURCES]
+    jobs += [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p),
+              "pairs_k%d_p%d" % (k, p)) for k, p in INSTANCES]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(lambda j: _compile(*j), jobs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

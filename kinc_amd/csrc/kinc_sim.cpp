@@ -23,9 +23,15 @@
 #include "../../include/kinc_sim.h"
 #include "sim_device.h"
 
+#define KINC_FOR_EACH_INSTANCE(X)                                                   \
+   X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(5, 10) X(5, 12)  \
+   X(5, 14) X(5, 16) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(8, 5) X(8, 6) X(8, 7) X(8, 8)  \
+   X(8, 10) X(8, 12) X(8, 14) X(8, 16)
+
 extern "C" {
-const void* kinc_select_pair_kernel_k5(int P, int persistent);
-const void* kinc_select_pair_kernel_k8(int P, int persistent);
+#define KINC_DECL(K, P) const void* kinc_pair_kernel_k##K##_p##P();
+KINC_FOR_EACH_INSTANCE(KINC_DECL)
+#undef KINC_DECL
 const void* kinc_kernel_count_valid();
 const void* kinc_kernel_chunk_max();
 const void* kinc_kernel_bin_pairs();
@@ -49,10 +55,13 @@ int slots_for(int n)
    return -1;
 }
 
-const void* kinc_select_pair_kernel(int P, int kmax, int persistent)
+/* The pair kernel for (P, KMAX); one instance serves both forms. */
+const void* kinc_select_pair_kernel(int P, int kmax)
 {
-   if ( kmax == 5 ) return kinc_select_pair_kernel_k5(P, persistent);
-   if ( kmax == 8 ) return kinc_select_pair_kernel_k8(P, persistent);
+#define KINC_PICK(K, PP)                                                      \
+   if ( kmax == K && P == PP ) return kinc_pair_kernel_k##K##_p##PP();
+   KINC_FOR_EACH_INSTANCE(KINC_PICK)
+#undef KINC_PICK
    return nullptr;
 }
 
@@ -277,6 +286,8 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    off = align16(off + (size_t)S);
    a.lds_w = (int32_t)off;
    off = align16(off + (size_t)S);
+   a.lds_prm = (int32_t)off;
+   off = align16(off + sizeof(float) * 8 * 8);
    a.lds_u = (int32_t)off;
    const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
    const size_t sort_bytes = sizeof(float) * 3 * (size_t)a.np2;
@@ -426,8 +437,9 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
    if ( h->p.form == KINC_FORM_PLAIN )
    {
       const int P = slots_for(h->S);
-      const void* fn = kinc_select_pair_kernel(P, h->kmax, 0);
+      const void* fn = kinc_select_pair_kernel(P, h->kmax);
       if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
+      a.list = nullptr;
       for ( int first = 0; first < cnt; first += h->p.ksize )
       {
          a.first = first;
@@ -460,7 +472,7 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
    for ( int b = h->nbins - 1; b >= 0; --b )
    {
       const int P = kSlotList[b];
-      const void* fn = kinc_select_pair_kernel(P, h->kmax, 1);
+      const void* fn = kinc_select_pair_kernel(P, h->kmax);
       if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
       HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));

@@ -53,6 +53,7 @@ struct SimArgs
    int32_t lds_x;            /* float2[S]  compact X, then the stale-read view */
    int32_t lds_lab;          /* int8[S]    full-index labels */
    int32_t lds_w;            /* int8[S]    work labels of the current fit */
+   int32_t lds_prm;          /* float[8][8] component parameters */
    int32_t lds_u;            /* union: chain-term tile / sort buffers */
    int32_t np2;              /* next power of two >= S (min 2) */
    int32_t tile_stride;      /* floats per chain row of the term tile */

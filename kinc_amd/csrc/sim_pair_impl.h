@@ -39,6 +39,13 @@
 #define WAVE 64
 #define TILE 64
 
+/* One wave per workgroup; the second bound asks for at least KINC_PAIR_WAVES
+ * waves per SIMD, which caps the VGPR budget of the pair kernels. */
+#ifndef KINC_PAIR_WAVES
+#define KINC_PAIR_WAVES 2
+#endif
+#define KINC_PAIR_BOUNDS __launch_bounds__(64, KINC_PAIR_WAVES)
+
 namespace {
 
 /* ------------------------------------------------------------ wave utils */
@@ -132,6 +139,7 @@ struct Lds
    float2* X;      /* [S] */
    int8_t* lab;    /* [S] */
    int8_t* w;      /* [S] */
+   float* prm;     /* [KMAX][8] component parameters / K-means means */
    float* u;       /* union region */
 };
 
@@ -142,6 +150,7 @@ __device__ __forceinline__ Lds lds_view(const SimArgs& a)
    l.X = (float2*)(lds_raw + a.lds_x);
    l.lab = (int8_t*)(lds_raw + a.lds_lab);
    l.w = (int8_t*)(lds_raw + a.lds_w);
+   l.prm = (float*)(lds_raw + a.lds_prm);
    l.u = (float*)(lds_raw + a.lds_u);
    return l;
 }
@@ -337,14 +346,16 @@ struct FitOut
    bool ok;
 };
 
-template <int KMAX>
-struct GmmParams
-{
-   float mu0[KMAX], mu1[KMAX];
-   float a00[KMAX], a01[KMAX], a10[KMAX], a11[KMAX];
-   float nrm[KMAX];
-   float logpi[KMAX];
-};
+/* Component parameters live in LDS, 8 floats each:
+ * mu0 mu1 a00 a01 | a10 a11 normalizer logpi (a = Sigma^-1). */
+#define PRM_MU0 0
+#define PRM_MU1 1
+#define PRM_A00 2
+#define PRM_A01 3
+#define PRM_A10 4
+#define PRM_A11 5
+#define PRM_NRM 6
+#define PRM_LPI 7
 
 /* pairwise_linalg.cpp:163-173 + pairwise_gmm.cpp:31-47; false when det <= 0. */
 __device__ __forceinline__ bool prepare_cov(float s00, float s01, float s10, float s11,
@@ -360,90 +371,108 @@ __device__ __forceinline__ bool prepare_cov(float s00, float s01, float s10, flo
    return true;
 }
 
-/* E step for all slots: lg[k][m] = log N(x|k) - log p(x), lpx[m] = log p(x)
- * (pairwise_gmm.cpp:54-77,143-187). */
-template <int P, int KMAX>
-__device__ __forceinline__ void gmm_estep(int N, int K, const float (&x0)[P], const float (&x1)[P],
-                                          const GmmParams<KMAX>& g, float (&lg)[KMAX][P], float (&lpx)[P])
+/* uniform value v_k of lane k, selected into lane k's register */
+template <int K>
+__device__ __forceinline__ float lane_pick(const float (&v)[K])
 {
    const int lane = lane_id();
+   float r = 0.0f;
+#pragma unroll
+   for ( int k = 0; k < K; ++k ) r = (lane == k) ? v[k] : r;
+   return r;
+}
+
+/* E step for the slots in use: lg[k][m] = log N(x|k) - log p(x),
+ * lpx[m] = log p(x) (pairwise_gmm.cpp:54-77,143-187).  Slots past the
+ * valid points compute harmless values that are never read. */
+template <int P, int K>
+__device__ __forceinline__ void gmm_estep(int nslots, const float (&x0)[P], const float (&x1)[P],
+                                          const float* prm, float (&lg)[K][P], float (&lpx)[P])
+{
+   float logpi[K];
+#pragma unroll
+   for ( int k = 0; k < K; ++k )
+   {
+      const float4 A = *(const float4*)(prm + 8 * k);
+      const float4 B = *(const float4*)(prm + 8 * k + 4);
+      logpi[k] = B.w;
+#pragma unroll
+      for ( int m = 0; m < P; ++m )
+      {
+         if ( m < nslots )
+         {
+            float xm0 = x0[m] - A.x;
+            float xm1 = x1[m] - A.y;
+            float b0 = A.z * xm0 + A.w * xm1;
+            float b1 = B.x * xm0 + B.y * xm1;
+            float dot = xm0 * b0 + xm1 * b1;
+            lg[k][m] = B.z - 0.5f * dot;
+         }
+      }
+   }
 #pragma unroll
    for ( int m = 0; m < P; ++m )
    {
-      if ( m * WAVE >= N ) break;
-      if ( lane + m * WAVE < N )
+      if ( m < nslots )
       {
-#pragma unroll
-         for ( int k = 0; k < KMAX; ++k )
-         {
-            if ( k < K )
-            {
-               float xm0 = x0[m] - g.mu0[k];
-               float xm1 = x1[m] - g.mu1[k];
-               float b0 = g.a00[k] * xm0 + g.a01[k] * xm1;
-               float b1 = g.a10[k] * xm0 + g.a11[k] * xm1;
-               float dot = xm0 * b0 + xm1 * b1;
-               lg[k][m] = g.nrm[k] - 0.5f * dot;
-            }
-         }
          float mx = -INFINITY;
 #pragma unroll
-         for ( int k = 0; k < KMAX; ++k )
+         for ( int k = 0; k < K; ++k )
          {
-            if ( k < K )
-            {
-               float a = g.logpi[k] + lg[k][m];
-               mx = (a > mx) ? a : mx;
-            }
+            float v = logpi[k] + lg[k][m];
+            mx = (v > mx) ? v : mx;
          }
          float sum = 0.0f;
 #pragma unroll
-         for ( int k = 0; k < KMAX; ++k )
-         {
-            if ( k < K ) sum = sum + kinc_expf((g.logpi[k] + lg[k][m]) - mx);
-         }
+         for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf((logpi[k] + lg[k][m]) - mx);
          float lp = mx + kinc_logf(sum);
          lpx[m] = lp;
 #pragma unroll
-         for ( int k = 0; k < KMAX; ++k )
-         {
-            if ( k < K ) lg[k][m] = lg[k][m] + (-lp);
-         }
+         for ( int k = 0; k < K; ++k ) lg[k][m] = lg[k][m] + (-lp);
       }
    }
 }
 
-/* GMM::fit (pairwise_gmm.cpp:367-445) with the per-fit LCG. Writes the
- * fit's labels for i < N into L.w. */
-template <int P, int KMAX>
-__device__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N, int K,
-                          const float (&x0)[P], const float (&x1)[P], bool want_entropy)
+/* GMM::fit (pairwise_gmm.cpp:367-445) with the per-fit LCG.  Writes the
+ * fit's labels for i < N into L.w.  Per-component scalar work of the M step
+ * runs with component k on lane k. */
+template <int P, int K>
+__device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
+                                          const float (&x0)[P], const float (&x1)[P], bool want_entropy)
 {
    const int lane = lane_id();
    const int STR = a.tile_stride;
+   const int nslots = (N + WAVE - 1) / WAVE;
    float* tile = L.u;
-   GmmParams<KMAX> g;
+   float* prm = L.prm;
    FitOut out;
    out.ok = true;
    out.logL = 0.0f;
    out.E = 0.0f;
 
-   uint64_t st = 1;
-#pragma unroll
-   for ( int k = 0; k < KMAX; ++k )
+   /* initialisation (pairwise_gmm.cpp:372-392): K draws of rand() % N */
    {
-      if ( k < K )
+      uint64_t st = 1;
+      int my = 0;
+#pragma unroll
+      for ( int k = 0; k < K; ++k )
       {
          int i = lcg(st) % N;
-         float2 c = L.X[i];
-         g.mu0[k] = c.x;
-         g.mu1[k] = c.y;
-         prepare_cov(1.0f, 0.0f, 0.0f, 1.0f, g.a00[k], g.a01[k], g.a10[k], g.a11[k], g.nrm[k]);
-         g.logpi[k] = kinc_logf(1.0f / K);
+         my = (lane == k) ? i : my;
       }
+      if ( lane < K )
+      {
+         float2 c = L.X[my];
+         float a00, a01, a10, a11, nrm;
+         prepare_cov(1.0f, 0.0f, 0.0f, 1.0f, a00, a01, a10, a11, nrm);
+         float* p = prm + 8 * lane;
+         *(float4*)p = make_float4(c.x, c.y, a00, a01);
+         *(float4*)(p + 4) = make_float4(a10, a11, nrm, kinc_logf(1.0f / K));
+      }
+      wave_sync();
    }
 
-   float lg[KMAX][P];
+   float lg[K][P];
    float lpx[P];
    float prevLogL = -INFINITY;
    float curLogL = -INFINITY;
@@ -453,54 +482,49 @@ __device__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N, int K,
 
    for ( int t = 0; t < 100; ++t )
    {
-      gmm_estep<P, KMAX>(N, K, x0, x1, g, lg, lpx);
+      gmm_estep<P, K>(nslots, x0, x1, prm, lg, lpx);
       e_steps++;
 
-      float gmax[KMAX];
+      float gmax[K];
 #pragma unroll
-      for ( int k = 0; k < KMAX; ++k )
+      for ( int k = 0; k < K; ++k )
       {
-         if ( k < K )
-         {
-            float v = -INFINITY;
+         float v = -INFINITY;
 #pragma unroll
-            for ( int m = 0; m < P; ++m )
+         for ( int m = 0; m < P; ++m )
+         {
+            if ( m < nslots )
             {
-               if ( lane + m * WAVE < N ) v = (lg[k][m] > v) ? lg[k][m] : v;
+               const bool valid = lane + m * WAVE < N;
+               v = (valid && lg[k][m] > v) ? lg[k][m] : v;
             }
-            gmax[k] = wave_max(v);
          }
+         gmax[k] = wave_max(v);
       }
 
       /* pass A: chain 0 = logL, 1+k = Gamma_k sum, 1+K+2k / 2+K+2k = mu_k sums */
-      const int nA = 1 + 3 * K;
       float acc = 0.0f;
 #pragma unroll
       for ( int m = 0; m < P; ++m )
       {
-         if ( m * WAVE >= N ) break;
-         if ( lane + m * WAVE < N )
+         if ( m < nslots )
          {
-            const int col = lane;
-            tile[col] = lpx[m];
+            tile[lane] = lpx[m];
 #pragma unroll
-            for ( int k = 0; k < KMAX; ++k )
+            for ( int k = 0; k < K; ++k )
             {
-               if ( k < K )
-               {
-                  tile[(1 + k) * STR + col] = kinc_expf(lg[k][m] - gmax[k]);
-                  float gm = kinc_expf(lg[k][m]);
-                  lg[k][m] = gm;
-                  tile[(1 + K + 2 * k) * STR + col] = gm * x0[m];
-                  tile[(2 + K + 2 * k) * STR + col] = gm * x1[m];
-               }
+               tile[(1 + k) * STR + lane] = kinc_expf(lg[k][m] - gmax[k]);
+               const float gm = kinc_expf(lg[k][m]);
+               lg[k][m] = gm;
+               tile[(1 + K + 2 * k) * STR + lane] = gm * x0[m];
+               tile[(2 + K + 2 * k) * STR + lane] = gm * x1[m];
             }
+            wave_sync();
+            int len = N - m * WAVE;
+            len = len < TILE ? len : TILE;
+            acc = chain_add(tile, STR, 1 + 3 * K, len, acc);
+            wave_sync();
          }
-         wave_sync();
-         int len = N - m * WAVE;
-         len = len < TILE ? len : TILE;
-         acc = chain_add(tile, STR, nA, len, acc);
-         wave_sync();
       }
 
       prevLogL = curLogL;
@@ -511,98 +535,95 @@ __device__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N, int K,
          break;
       }
 
-      /* calcLogGammaK / calcLogGammaSum (pairwise_gmm.cpp:194-248) */
-      float logGamma[KMAX];
-#pragma unroll
-      for ( int k = 0; k < KMAX; ++k )
-      {
-         if ( k < K ) logGamma[k] = gmax[k] + kinc_logf(bcast(acc, 1 + k));
-      }
+      /* calcLogGammaK / calcLogGammaSum / performMStep (pairwise_gmm.cpp:194-316):
+       * component k on lane k */
+      const float gsum = __shfl(acc, 1 + lane);
+      const float mxs = __shfl(acc, 1 + K + 2 * lane);
+      const float mys = __shfl(acc, 2 + K + 2 * lane);
+      const float logGamma = lane_pick<K>(gmax) + kinc_logf(gsum);
+      const float lpk = prm[8 * (lane < K ? lane : 0) + PRM_LPI];
+      const float v = lpk + logGamma;
       float mx = -INFINITY;
 #pragma unroll
-      for ( int k = 0; k < KMAX; ++k )
+      for ( int k = 0; k < K; ++k )
       {
-         if ( k < K )
-         {
-            float v = g.logpi[k] + logGamma[k];
-            mx = (v > mx) ? v : mx;
-         }
+         const float vk = bcast(v, k);
+         mx = (vk > mx) ? vk : mx;
       }
       float sum = 0.0f;
 #pragma unroll
-      for ( int k = 0; k < KMAX; ++k )
-      {
-         if ( k < K ) sum = sum + kinc_expf((g.logpi[k] + logGamma[k]) - mx);
-      }
+      for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf(bcast(v, k) - mx);
       const float logGammaSum = mx + kinc_logf(sum);
 
-      /* performMStep (pairwise_gmm.cpp:255-316) */
-      float Gam[KMAX];
-#pragma unroll
-      for ( int k = 0; k < KMAX; ++k )
+      const float Gam = kinc_expf(logGamma);
+      const float cm = 1.0f / Gam;
+      const float mu0 = mxs * cm;
+      const float mu1 = mys * cm;
+      wave_sync();
+      if ( lane < K )
       {
-         if ( k < K )
-         {
-            g.logpi[k] = g.logpi[k] + (logGamma[k] - logGammaSum);
-            Gam[k] = kinc_expf(logGamma[k]);
-            const float cm = 1.0f / Gam[k];
-            g.mu0[k] = bcast(acc, 1 + K + 2 * k) * cm;
-            g.mu1[k] = bcast(acc, 2 + K + 2 * k) * cm;
-         }
+         float* p = prm + 8 * lane;
+         p[PRM_MU0] = mu0;
+         p[PRM_MU1] = mu1;
+         p[PRM_LPI] = lpk + (logGamma - logGammaSum);
       }
+      wave_sync();
 
       /* pass B: chain 3k / 3k+1 / 3k+2 = Sigma_k entries 00, 01 (= 10), 11 */
-      const int nB = 3 * K;
       float accB = 0.0f;
+      float muk0[K], muk1[K];
+#pragma unroll
+      for ( int k = 0; k < K; ++k )
+      {
+         const float2 mu = *(const float2*)(prm + 8 * k);
+         muk0[k] = mu.x;
+         muk1[k] = mu.y;
+      }
 #pragma unroll
       for ( int m = 0; m < P; ++m )
       {
-         if ( m * WAVE >= N ) break;
-         if ( lane + m * WAVE < N )
+         if ( m < nslots )
          {
-            const int col = lane;
 #pragma unroll
-            for ( int k = 0; k < KMAX; ++k )
+            for ( int k = 0; k < K; ++k )
             {
-               if ( k < K )
-               {
-                  float xm0 = x0[m] - g.mu0[k];
-                  float xm1 = x1[m] - g.mu1[k];
-                  float gm = lg[k][m];
-                  tile[(3 * k) * STR + col] = gm * (xm0 * xm0);
-                  tile[(3 * k + 1) * STR + col] = gm * (xm0 * xm1);
-                  tile[(3 * k + 2) * STR + col] = gm * (xm1 * xm1);
-               }
+               const float xm0 = x0[m] - muk0[k];
+               const float xm1 = x1[m] - muk1[k];
+               const float gm = lg[k][m];
+               tile[(3 * k) * STR + lane] = gm * (xm0 * xm0);
+               tile[(3 * k + 1) * STR + lane] = gm * (xm0 * xm1);
+               tile[(3 * k + 2) * STR + lane] = gm * (xm1 * xm1);
             }
+            wave_sync();
+            int len = N - m * WAVE;
+            len = len < TILE ? len : TILE;
+            accB = chain_add(tile, STR, 3 * K, len, accB);
+            wave_sync();
          }
-         wave_sync();
-         int len = N - m * WAVE;
-         len = len < TILE ? len : TILE;
-         accB = chain_add(tile, STR, nB, len, accB);
-         wave_sync();
       }
 
-#pragma unroll
-      for ( int k = 0; k < KMAX; ++k )
-      {
-         if ( k < K && !failed )
-         {
-            const float cs = 1.0f / Gam[k];
-            float s00 = bcast(accB, 3 * k) * cs;
-            float s01 = bcast(accB, 3 * k + 1) * cs;
-            float s11 = bcast(accB, 3 * k + 2) * cs;
-            if ( !prepare_cov(s00, s01, s01, s11, g.a00[k], g.a01[k], g.a10[k], g.a11[k], g.nrm[k]) )
-            {
-               failed = true;
-            }
-         }
-      }
+      const float s00 = __shfl(accB, 3 * lane) * cm;
+      const float s01 = __shfl(accB, 3 * lane + 1) * cm;
+      const float s11 = __shfl(accB, 3 * lane + 2) * cm;
+      float a00, a01, a10, a11, nrm = 0.0f;
+      const bool ok = prepare_cov(s00, s01, s01, s11, a00, a01, a10, a11, nrm);
+      failed = ballot(lane < K && !ok) != 0;
       if ( failed ) break;
+      if ( lane < K )
+      {
+         float* p = prm + 8 * lane;
+         *(float4*)(p + 0) = make_float4(mu0, mu1, a00, a01);
+         p[PRM_A10] = a10;
+         p[PRM_A11] = a11;
+         p[PRM_NRM] = nrm;
+      }
+      wave_sync();
    }
 
    out.iters = (uint16_t)((e_steps & 0x0fff) | (converged ? (1u << 12) : 0u) | (failed ? (1u << 13) : 0u));
    if ( failed )
    {
+      wave_sync();
       out.ok = false;
       return out;
    }
@@ -613,7 +634,7 @@ __device__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N, int K,
     * that ran 100 iterations labels from the exponentiated ones. */
    if ( converged )
    {
-      gmm_estep<P, KMAX>(N, K, x0, x1, g, lg, lpx);
+      gmm_estep<P, K>(nslots, x0, x1, prm, lg, lpx);
    }
 
    /* calcLabels (pairwise_gmm.cpp:323-341) and the entropy terms (:348-360) */
@@ -621,25 +642,23 @@ __device__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N, int K,
 #pragma unroll
    for ( int m = 0; m < P; ++m )
    {
-      int i = lane + m * WAVE;
+      const int i = lane + m * WAVE;
       ent[m] = 0.0f;
-      if ( i < N )
+      if ( m < nslots )
       {
          int max_k = -1;
          float mg = -INFINITY;
-         float sel = NAN;
 #pragma unroll
-         for ( int k = 0; k < KMAX; ++k )
+         for ( int k = 0; k < K; ++k )
          {
-            if ( k < K && mg < lg[k][m] )
+            if ( mg < lg[k][m] )
             {
                max_k = k;
                mg = lg[k][m];
-               sel = lg[k][m];
             }
          }
-         L.w[i] = (int8_t)max_k;
-         ent[m] = (max_k >= 0) ? kinc_logf(sel) : NAN;
+         if ( i < N ) L.w[i] = (int8_t)max_k;
+         if ( want_entropy ) ent[m] = (max_k >= 0) ? kinc_logf(mg) : NAN;
       }
    }
    wave_sync();
@@ -650,13 +669,15 @@ __device__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N, int K,
 #pragma unroll
       for ( int m = 0; m < P; ++m )
       {
-         if ( m * WAVE >= N ) break;
-         if ( lane + m * WAVE < N ) tile[lane] = ent[m];
-         wave_sync();
-         int len = N - m * WAVE;
-         len = len < TILE ? len : TILE;
-         accE = chain_add(tile, STR, 1, len, accE);
-         wave_sync();
+         if ( m < nslots )
+         {
+            tile[lane] = ent[m];
+            wave_sync();
+            int len = N - m * WAVE;
+            len = len < TILE ? len : TILE;
+            accE = chain_add(tile, STR, 1, len, accE);
+            wave_sync();
+         }
       }
       out.E = bcast(accE, 0);
    }
@@ -674,67 +695,73 @@ __device__ __forceinline__ float diff_norm(float ax, float ay, float bx, float b
    return sqrtf(dist);
 }
 
-/* KMeans::fit (pairwise_kmeans.cpp:12-127) with the per-fit LCG. */
-template <int P, int KMAX>
-__device__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N, int K,
-                             const float (&x0)[P], const float (&x1)[P])
+/* KMeans::fit (pairwise_kmeans.cpp:12-127) with the per-fit LCG.  Means live
+ * in LDS (2 floats per cluster); the best init's labels go to L.w. */
+template <int P, int K>
+__device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N,
+                                             const float (&x0)[P], const float (&x1)[P])
 {
    const int lane = lane_id();
    const int STR = a.tile_stride;
+   const int nslots = (N + WAVE - 1) / WAVE;
    float* tile = L.u;
+   float* means = L.prm;
    FitOut out;
    out.ok = true;
    out.E = 0.0f;
    uint64_t st = 1;
    float bestLogL = -INFINITY;
    int iters = 0;
-   float m0[KMAX], m1[KMAX];
    int8_t y[P];
 
    for ( int init = 0; init < 10; ++init )
    {
+      int my = 0;
 #pragma unroll
-      for ( int k = 0; k < KMAX; ++k )
+      for ( int k = 0; k < K; ++k )
       {
-         if ( k < K )
-         {
-            int i = lcg(st) % N;
-            float2 c = L.X[i];
-            m0[k] = c.x;
-            m1[k] = c.y;
-         }
+         int i = lcg(st) % N;
+         my = (lane == k) ? i : my;
       }
+      wave_sync();
+      if ( lane < K ) *(float2*)(means + 2 * lane) = L.X[my];
+      wave_sync();
 #pragma unroll
       for ( int m = 0; m < P; ++m ) y[m] = 0;
 
       for ( int t = 0; t < 300; ++t )
       {
          iters++;
+         float m0[K], m1[K];
+#pragma unroll
+         for ( int k = 0; k < K; ++k )
+         {
+            const float2 mk = *(const float2*)(means + 2 * k);
+            m0[k] = mk.x;
+            m1[k] = mk.y;
+         }
          int8_t yn[P];
          bool diff = false;
 #pragma unroll
          for ( int m = 0; m < P; ++m )
          {
             yn[m] = 0;
-            if ( lane + m * WAVE < N )
+            if ( m < nslots )
             {
                int min_k = -1;
                float min_d = 0.0f;
 #pragma unroll
-               for ( int k = 0; k < KMAX; ++k )
+               for ( int k = 0; k < K; ++k )
                {
-                  if ( k < K )
+                  const float d = diff_norm(x0[m], x1[m], m0[k], m1[k]);
+                  if ( min_k == -1 || d < min_d )
                   {
-                     float d = diff_norm(x0[m], x1[m], m0[k], m1[k]);
-                     if ( min_k == -1 || d < min_d )
-                     {
-                        min_k = k;
-                        min_d = d;
-                     }
+                     min_k = k;
+                     min_d = d;
                   }
                }
                yn[m] = (int8_t)min_k;
-               diff = diff || (yn[m] != y[m]);
+               diff = diff || ((lane + m * WAVE < N) && yn[m] != y[m]);
             }
          }
          if ( ballot(diff) == 0 ) break;
@@ -742,47 +769,43 @@ __device__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N, int K,
          for ( int m = 0; m < P; ++m ) y[m] = yn[m];
 
          /* means: chain 2k / 2k+1 over members in index order */
-         const int nch = 2 * K;
          float acc = 0.0f;
 #pragma unroll
          for ( int m = 0; m < P; ++m )
          {
-            if ( m * WAVE >= N ) break;
-            if ( lane + m * WAVE < N )
+            if ( m < nslots )
             {
 #pragma unroll
-               for ( int k = 0; k < KMAX; ++k )
+               for ( int k = 0; k < K; ++k )
                {
-                  if ( k < K )
-                  {
-                     bool mem = y[m] == k;
-                     tile[(2 * k) * STR + lane] = mem ? x0[m] : 0.0f;
-                     tile[(2 * k + 1) * STR + lane] = mem ? x1[m] : 0.0f;
-                  }
+                  const bool mem = y[m] == k;
+                  tile[(2 * k) * STR + lane] = mem ? x0[m] : 0.0f;
+                  tile[(2 * k + 1) * STR + lane] = mem ? x1[m] : 0.0f;
                }
+               wave_sync();
+               int len = N - m * WAVE;
+               len = len < TILE ? len : TILE;
+               acc = chain_add(tile, STR, 2 * K, len, acc);
+               wave_sync();
             }
-            wave_sync();
-            int len = N - m * WAVE;
-            len = len < TILE ? len : TILE;
-            acc = chain_add(tile, STR, nch, len, acc);
-            wave_sync();
          }
+         int nk = 0;
 #pragma unroll
-         for ( int k = 0; k < KMAX; ++k )
+         for ( int k = 0; k < K; ++k )
          {
-            if ( k < K )
-            {
-               int nk = 0;
+            int cnt = 0;
 #pragma unroll
-               for ( int m = 0; m < P; ++m )
-               {
-                  nk += popc(ballot(lane + m * WAVE < N && y[m] == k));
-               }
-               const float c = 1.0f / (float)nk;
-               m0[k] = bcast(acc, 2 * k) * c;
-               m1[k] = bcast(acc, 2 * k + 1) * c;
+            for ( int m = 0; m < P; ++m )
+            {
+               if ( m < nslots ) cnt += popc(ballot(lane + m * WAVE < N && y[m] == k));
             }
+            nk = (lane == k) ? cnt : nk;
          }
+         const float sx = __shfl(acc, 2 * lane);
+         const float sy = __shfl(acc, 2 * lane + 1);
+         const float c = 1.0f / (float)nk;
+         if ( lane < K ) *(float2*)(means + 2 * lane) = make_float2(sx * c, sy * c);
+         wave_sync();
       }
 
       /* computeLogLikelihood (pairwise_kmeans.cpp:106-127): one chain over
@@ -790,25 +813,26 @@ __device__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N, int K,
       float* seq = L.u;
       int kbase = 0;
 #pragma unroll
-      for ( int k = 0; k < KMAX; ++k )
+      for ( int k = 0; k < K; ++k )
       {
-         if ( k < K )
-         {
-            int cnt = 0;
+         const float2 mk = *(const float2*)(means + 2 * k);
+         int cnt = 0;
 #pragma unroll
-            for ( int m = 0; m < P; ++m )
+         for ( int m = 0; m < P; ++m )
+         {
+            if ( m < nslots )
             {
-               bool mem = (lane + m * WAVE < N) && y[m] == k;
-               uint64_t bm = ballot(mem);
+               const bool mem = (lane + m * WAVE < N) && y[m] == k;
+               const uint64_t bm = ballot(mem);
                if ( mem )
                {
-                  float d = diff_norm(x0[m], x1[m], m0[k], m1[k]);
+                  const float d = diff_norm(x0[m], x1[m], mk.x, mk.y);
                   seq[kbase + cnt + popc(bm & lanes_below())] = d * d;
                }
                cnt += popc(bm);
             }
-            kbase += cnt;
          }
+         kbase += cnt;
       }
       wave_sync();
       float Ssum = 0.0f;
@@ -816,8 +840,8 @@ __device__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N, int K,
       {
          for ( int j = 0; j < kbase; ++j ) Ssum = Ssum + seq[j];
       }
-      wave_sync();
       const float logL = -bcast(Ssum, 0);
+      wave_sync();
 
       if ( bestLogL < logL )
       {
@@ -825,8 +849,8 @@ __device__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N, int K,
 #pragma unroll
          for ( int m = 0; m < P; ++m )
          {
-            int i = lane + m * WAVE;
-            if ( i < N ) L.w[i] = y[m];
+            const int i = lane + m * WAVE;
+            if ( m < nslots && i < N ) L.w[i] = y[m];
          }
          wave_sync();
       }
@@ -1026,8 +1050,48 @@ __device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
 
 /* -------------------------------------------------------------- driver */
 
+/* One clustering fit with Kc components, Kc <= KMAX (compile-time K). */
+template <int P, int K>
+__device__ __forceinline__ FitOut fit_one(const SimArgs& a, const Lds& L, int N,
+                                          const float (&x0)[P], const float (&x1)[P])
+{
+   return (a.clus == 1) ? gmm_fit<P, K>(a, L, N, x0, x1, a.crit == 1)
+                        : kmeans_fit<P, K>(a, L, N, x0, x1);
+}
+
 template <int P, int KMAX>
-__device__ void process_pair(const SimArgs& a, int q)
+__device__ __forceinline__ FitOut fit_dispatch(const SimArgs& a, const Lds& L, int N, int Kc,
+                                               const float (&x0)[P], const float (&x1)[P])
+{
+   switch ( Kc )
+   {
+   case 1: return fit_one<P, 1>(a, L, N, x0, x1);
+   case 2: return fit_one<P, 2>(a, L, N, x0, x1);
+   case 3: return fit_one<P, 3>(a, L, N, x0, x1);
+   case 4: return fit_one<P, 4>(a, L, N, x0, x1);
+   case 5: return fit_one<P, 5>(a, L, N, x0, x1);
+   default: break;
+   }
+   if ( KMAX > 5 )
+   {
+      switch ( Kc )
+      {
+      case 6: return fit_one<P, (KMAX > 5 ? 6 : 1)>(a, L, N, x0, x1);
+      case 7: return fit_one<P, (KMAX > 5 ? 7 : 1)>(a, L, N, x0, x1);
+      case 8: return fit_one<P, (KMAX > 5 ? 8 : 1)>(a, L, N, x0, x1);
+      default: break;
+      }
+   }
+   FitOut f;
+   f.ok = false;
+   f.iters = 0;
+   f.logL = 0.0f;
+   f.E = 0.0f;
+   return f;
+}
+
+template <int P, int KMAX>
+__device__ __forceinline__ void process_pair(const SimArgs& a, int q)
 {
    const Lds L = lds_view(a);
    const int lane = lane_id();
@@ -1064,9 +1128,7 @@ __device__ void process_pair(const SimArgs& a, int q)
          float bestValue = INFINITY;
          for ( int Kc = a.min_k; Kc <= a.max_k; ++Kc )
          {
-            FitOut f = (a.clus == 1)
-               ? gmm_fit<P, KMAX>(a, L, N, Kc, x0, x1, a.crit == 1)
-               : kmeans_fit<P, KMAX>(a, L, N, Kc, x0, x1);
+            FitOut f = fit_dispatch<P, KMAX>(a, L, N, Kc, x0, x1);
             if ( lane == Kc - 1 ) itv = f.iters;
             if ( !f.ok ) continue;
 
@@ -1134,17 +1196,21 @@ __device__ void process_pair(const SimArgs& a, int q)
 
 /* ------------------------------------------------------------- kernels */
 
+/*
+ * The pair kernel of both execution forms.  Plain form (a.list == null): one
+ * workgroup per pair of the launch.  Persistent form: resident waves pop
+ * block-relative pair ids from a device queue until it is empty, so every
+ * wave reaches the exit test.
+ */
 template <int P, int KMAX>
-__global__ __launch_bounds__(64) void k_pairs_plain(SimArgs a)
+__global__ KINC_PAIR_BOUNDS void k_pairs(SimArgs a)
 {
-   const int q = a.first + (int)blockIdx.x;
-   if ( q >= a.first + a.launch_count ) return;
-   process_pair<P, KMAX>(a, q);
-}
-
-template <int P, int KMAX>
-__global__ __launch_bounds__(64) void k_pairs_persistent(SimArgs a)
-{
+   if ( a.list == nullptr )
+   {
+      const int q = a.first + (int)blockIdx.x;
+      if ( q < a.first + a.launch_count ) process_pair<P, KMAX>(a, q);
+      return;
+   }
    const int len = *a.list_len;
    for ( ;; )
    {
