@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "gene pairs/sec GMM+Spearman 20k×500, batched vs plain; p50/p99 query latency"
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per launch of the dominant pair kernel, from the FETCH_SIZE and
+# WRITE_SIZE passes (scaled by a copy kernel of known size) recorded in:
+TRAFFIC_SOURCE = "profiles/k_pairs_gmm_spearman_20000x500_counters.md"
+TRAFFIC_BYTES_PER_LAUNCH = 349e6
 BLOCK = 32768
 
 
@@ -271,7 +275,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "traffic_source": TRAFFIC_SOURCE,
             "whole_job_tflops": whole_tflops,
             "kernel_time_share": kernel_ms_total / (secs * 1e3),
         },

@@ -117,7 +117,11 @@ int kinc_sim_execute(kinc_sim* h, int64_t start, int64_t count,
 /*
  * Same as kinc_sim_execute but the outputs are device pointers on the
  * handle's device and the call is asynchronous on `stream` (a hipStream_t,
- * or NULL for the handle's own stream).  labels and iters may be NULL.
+ * or NULL for the handle's own stream): the work is ordered after what is
+ * already queued on `stream`, and what is queued on `stream` afterwards
+ * runs after it.  labels and iters may be NULL.  Calls on one handle share
+ * its per-block scratch, so they must be issued on one stream (or otherwise
+ * ordered).
  */
 int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count,
                             int8_t* K, int8_t* labels, float* corr,

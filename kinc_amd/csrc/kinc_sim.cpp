@@ -101,6 +101,11 @@ struct kinc_sim
    int8_t* d_labels = nullptr;
    float* d_corr = nullptr;
    uint16_t* d_iters = nullptr;
+   /* the persistent form drains its bins concurrently on these streams */
+   static const int kAux = 4;
+   hipStream_t aux[kAux] = {};
+   hipEvent_t binned = nullptr;
+   hipEvent_t drained[kAux] = {};
    SimArgs base{};
    size_t lds_bytes = 0;
    std::string err;
@@ -234,6 +239,12 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    CREATE_TRY(hipGetDeviceProperties(&prop, device));
    h->num_cus = prop.multiProcessorCount;
    CREATE_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+   for ( int i = 0; i < kinc_sim::kAux; ++i )
+   {
+      CREATE_TRY(hipStreamCreateWithFlags(&h->aux[i], hipStreamNonBlocking));
+      CREATE_TRY(hipEventCreateWithFlags(&h->drained[i], hipEventDisableTiming));
+   }
+   CREATE_TRY(hipEventCreateWithFlags(&h->binned, hipEventDisableTiming));
 
    const size_t emx_bytes = (size_t)G * (size_t)S * sizeof(float);
    CREATE_TRY(hipMalloc(&h->d_emx, emx_bytes));
@@ -469,9 +480,14 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
       int rc = launch(h, kinc_kernel_bin_pairs(), (unsigned)((cnt + 255) / 256), 256, args, 0, s);
       if ( rc ) return rc;
    }
+   /* Bins run concurrently: a bin with few pairs would otherwise leave most
+    * of the device idle for one pair's latency (milliseconds). */
+   HIP_TRY(h, hipEventRecord(h->binned, s));
+   for ( int i = 0; i < kinc_sim::kAux; ++i ) HIP_TRY(h, hipStreamWaitEvent(h->aux[i], h->binned, 0));
    for ( int b = h->nbins - 1; b >= 0; --b )
    {
       const int P = kSlotList[b];
+      hipStream_t sb = h->aux[(h->nbins - 1 - b) % kinc_sim::kAux];
       const void* fn = kinc_select_pair_kernel(P, h->kmax);
       if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
@@ -483,8 +499,13 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
       a.list_len = h->d_counters + b;
       a.list_head = h->d_counters + kNumSlots + b;
       void* args[] = {&a};
-      int rc = launch_pairs(h, P, fn, (unsigned)grid, args, s);
+      int rc = launch_pairs(h, P, fn, (unsigned)grid, args, sb);
       if ( rc ) return rc;
+   }
+   for ( int i = 0; i < kinc_sim::kAux; ++i )
+   {
+      HIP_TRY(h, hipEventRecord(h->drained[i], h->aux[i]));
+      HIP_TRY(h, hipStreamWaitEvent(s, h->drained[i], 0));
    }
    return KINC_OK;
 }
@@ -513,6 +534,8 @@ extern "C" int kinc_sim_execute(kinc_sim* h, int64_t start, int64_t count,
 extern "C" int kinc_sim_synchronize(kinc_sim* h)
 {
    if ( !h ) return KINC_E_INVALID;
+   HIP_TRY(h, hipSetDevice(h->device));
+   for ( int i = 0; i < kinc_sim::kAux; ++i ) HIP_TRY(h, hipStreamSynchronize(h->aux[i]));
    HIP_TRY(h, hipStreamSynchronize(h->stream));
    return KINC_OK;
 }
@@ -539,6 +562,12 @@ extern "C" void kinc_sim_destroy(kinc_sim* h)
       (void)hipEventDestroy(t.b);
    }
    for ( auto e : h->event_pool ) (void)hipEventDestroy(e);
+   for ( int i = 0; i < kinc_sim::kAux; ++i )
+   {
+      if ( h->aux[i] ) (void)hipStreamDestroy(h->aux[i]);
+      if ( h->drained[i] ) (void)hipEventDestroy(h->drained[i]);
+   }
+   if ( h->binned ) (void)hipEventDestroy(h->binned);
    if ( h->stream ) (void)hipStreamDestroy(h->stream);
    delete h;
 }
