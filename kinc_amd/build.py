@@ -35,9 +35,10 @@ SOURCES = [
     "kinc_sim.cpp",
     "formats.cpp",
 ]
-# pair-kernel instances: (KMAX, P) for every register-slot count P
-SLOTS = (1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16)
-INSTANCES = [(k, p) for k in (5, 8) for p in SLOTS]
+# pair-kernel instances (KMAX, P): register-slot counts P, and P = 0 for the
+# memory-store instance used above 1024 samples
+INSTANCES = ([(5, p) for p in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16)]
+             + [(8, p) for p in (0, 2, 4, 8, 16)])
 CLI_SOURCE = "kinc_cli.cpp"
 CLI = os.path.join(HERE, "kinc")
 

@@ -24,9 +24,8 @@
 #include "sim_device.h"
 
 #define KINC_FOR_EACH_INSTANCE(X)                                                   \
-   X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(5, 10) X(5, 12)  \
-   X(5, 14) X(5, 16) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(8, 5) X(8, 6) X(8, 7) X(8, 8)  \
-   X(8, 10) X(8, 12) X(8, 14) X(8, 16)
+   X(5, 0) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(5, 10)   \
+   X(5, 12) X(5, 14) X(5, 16) X(8, 0) X(8, 2) X(8, 4) X(8, 8) X(8, 16)
 
 extern "C" {
 #define KINC_DECL(K, P) const void* kinc_pair_kernel_k##K##_p##P();
@@ -40,19 +39,32 @@ const void* kinc_kernel_zero_i32();
 
 namespace {
 
-/* Register-slot counts that have a kernel instance (64*P samples per pair). */
-const int kSlotList[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16};
-const int kNumSlots = sizeof(kSlotList) / sizeof(kSlotList[0]);
+/* Register-slot counts that have a kernel instance (64*P samples per pair),
+ * per cluster bound KMAX. */
+const int kSlotList5[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16};
+const int kSlotList8[] = {2, 4, 8, 16};
+const int kMaxBins = 12;
 
 thread_local std::string g_create_error;
 
-int slots_for(int n)
+/* Slot classes used for a handle: every instance up to the one S needs,
+ * or the memory-store instance (P = 0) alone above the register range. */
+std::vector<int> slot_classes(int S, int kmax)
 {
-   for ( int i = 0; i < kNumSlots; ++i )
+   std::vector<int> out;
+   if ( S > KINC_MAX_REG_SAMPLES )
    {
-      if ( 64 * kSlotList[i] >= n ) return kSlotList[i];
+      out.push_back(0);
+      return out;
    }
-   return -1;
+   const int* list = kmax == 5 ? kSlotList5 : kSlotList8;
+   const int n = kmax == 5 ? 12 : 4;
+   for ( int i = 0; i < n; ++i )
+   {
+      out.push_back(list[i]);
+      if ( 64 * list[i] >= S ) break;
+   }
+   return out;
 }
 
 /* The pair kernel for (P, KMAX); one instance serves both forms. */
@@ -92,10 +104,13 @@ struct kinc_sim
    double* d_logN = nullptr;
    int32_t* d_blockN = nullptr;
    int32_t* d_chunkMax = nullptr;
-   int32_t* d_counters = nullptr;  /* [2*kNumSlots]: bin sizes, then pop heads */
-   int32_t* d_lists = nullptr;     /* [kNumSlots][KINC_WORK_BLOCK_SIZE] */
+   int32_t* d_counters = nullptr;  /* [2*kMaxBins]: bin sizes, then pop heads */
+   int32_t* d_lists = nullptr;     /* [kMaxBins][KINC_WORK_BLOCK_SIZE] */
    int32_t* d_binP = nullptr;
+   std::vector<int> classes;       /* slot count of each bin (0 = memory store) */
    int nbins = 0;
+   char* d_scratch = nullptr;      /* memory-store slices */
+   int scratch_slots = 0;
    /* staging for the host-buffer entry point */
    int8_t* d_K = nullptr;
    int8_t* d_labels = nullptr;
@@ -196,7 +211,7 @@ static int validate(const kinc_sim_params* p, int G, int S, std::string& why)
    }
    if ( S > KINC_MAX_SAMPLES )
    {
-      why = "sample_size > " + std::to_string(KINC_MAX_SAMPLES) + " is not built on the device path";
+      why = "sample_size > " + std::to_string(KINC_MAX_SAMPLES) + " is not supported";
       return KINC_E_UNSUPPORTED;
    }
    return KINC_OK;
@@ -260,18 +275,19 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    const size_t B = KINC_WORK_BLOCK_SIZE;
    CREATE_TRY(hipMalloc(&h->d_blockN, sizeof(int32_t) * B));
    CREATE_TRY(hipMalloc(&h->d_chunkMax, sizeof(int32_t) * (B / 64 + 1)));
-   CREATE_TRY(hipMalloc(&h->d_counters, sizeof(int32_t) * 2 * kNumSlots));
-   CREATE_TRY(hipMalloc(&h->d_lists, sizeof(int32_t) * B * kNumSlots));
+   CREATE_TRY(hipMalloc(&h->d_counters, sizeof(int32_t) * 2 * kMaxBins));
+   CREATE_TRY(hipMalloc(&h->d_lists, sizeof(int32_t) * B * kMaxBins));
 
-   /* bins: every slot count up to the one S needs */
+   /* bins: every register-slot count up to the one S needs, or the single
+    * memory-store class; a bin takes pairs with 64*P >= N (the memory-store
+    * bin takes every pair) */
+   h->classes = slot_classes(S, h->kmax);
+   h->nbins = (int)h->classes.size();
    std::vector<int32_t> binP;
-   const int Pmax = slots_for(S);
-   for ( int i = 0; i < kNumSlots && kSlotList[i] <= Pmax; ++i ) binP.push_back(kSlotList[i]);
-   h->nbins = (int)binP.size();
+   for ( int P : h->classes ) binP.push_back(P > 0 ? P : (1 << 20));
    CREATE_TRY(hipMalloc(&h->d_binP, sizeof(int32_t) * binP.size()));
    CREATE_TRY(hipMemcpy(h->d_binP, binP.data(), sizeof(int32_t) * binP.size(), hipMemcpyHostToDevice));
 
-   /* LDS carve-up */
    SimArgs& a = h->base;
    a.emx = h->d_emx;
    a.G = G;
@@ -290,6 +306,11 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    a.chunkMax = h->d_chunkMax;
    a.np2 = next_pow2(S);
    a.tile_stride = 64 + 4;
+   a.mem_slots = (S + 63) / 64;
+   const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
+   const size_t sort_bytes = sizeof(float) * 3 * (size_t)a.np2;
+
+   /* per-pair arrays: X, labels, work labels, then sort buffers */
    size_t off = 0;
    a.lds_x = (int32_t)off;
    off = align16(off + sizeof(float) * 2 * (size_t)S);
@@ -297,13 +318,38 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    off = align16(off + (size_t)S);
    a.lds_w = (int32_t)off;
    off = align16(off + (size_t)S);
-   a.lds_prm = (int32_t)off;
-   off = align16(off + sizeof(float) * 8 * 8);
-   a.lds_u = (int32_t)off;
-   const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
-   const size_t sort_bytes = sizeof(float) * 3 * (size_t)a.np2;
-   off += std::max(tile_bytes, sort_bytes);
-   h->lds_bytes = align16(off);
+   if ( S <= KINC_MAX_REG_SAMPLES )
+   {
+      /* everything in LDS; the chain tile shares the sort-buffer region */
+      a.scratch = nullptr;
+      a.lds_prm = (int32_t)off;
+      off = align16(off + sizeof(float) * 8 * 8);
+      a.lds_u = (int32_t)off;
+      a.lds_tile = a.lds_u;
+      off += std::max(tile_bytes, sort_bytes);
+      h->lds_bytes = align16(off);
+   }
+   else
+   {
+      /* memory store: per-pair arrays in a scratch slice per workgroup, the
+       * parameters and the chain tile in LDS */
+      a.lds_u = (int32_t)off;
+      off = align16(off + sort_bytes);
+      a.scr_lg = (int32_t)off;
+      off = align16(off + sizeof(float) * (size_t)(h->kmax + 1) * (size_t)a.mem_slots * 64);
+      a.scr_y = (int32_t)off;
+      off = align16(off + (size_t)2 * (size_t)a.mem_slots * 64);
+      a.scratch_stride = (int64_t)((off + 255) & ~(size_t)255);
+      a.lds_prm = 0;
+      a.lds_tile = (int32_t)align16(sizeof(float) * 8 * 8);
+      h->lds_bytes = align16((size_t)a.lds_tile + tile_bytes);
+      const void* fn = kinc_select_pair_kernel(0, h->kmax);
+      int per_cu = 0;
+      CREATE_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
+      h->scratch_slots = std::max(1, per_cu) * h->num_cus;
+      CREATE_TRY(hipMalloc(&h->d_scratch, (size_t)h->scratch_slots * (size_t)a.scratch_stride));
+      a.scratch = h->d_scratch;
+   }
 
    *out = h;
    return KINC_OK;
@@ -447,7 +493,8 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
 
    if ( h->p.form == KINC_FORM_PLAIN )
    {
-      const int P = slots_for(h->S);
+      /* the slot count S needs (or the memory store) for every pair */
+      const int P = h->classes.back();
       const void* fn = kinc_select_pair_kernel(P, h->kmax);
       if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       a.list = nullptr;
@@ -455,8 +502,10 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
       {
          a.first = first;
          a.launch_count = std::min(h->p.ksize, cnt - first);
+         unsigned grid = (unsigned)a.launch_count;
+         if ( P == 0 ) grid = (unsigned)std::min(a.launch_count, h->scratch_slots);
          void* args[] = {&a};
-         int rc = launch_pairs(h, P, fn, (unsigned)a.launch_count, args, s);
+         int rc = launch_pairs(h, P, fn, grid, args, s);
          if ( rc ) return rc;
       }
       return KINC_OK;
@@ -465,7 +514,7 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
    /* persistent: bin by slot count, then drain each bin with resident waves */
    {
       int32_t* ctr = h->d_counters;
-      int n = 2 * kNumSlots;
+      int n = 2 * kMaxBins;
       void* args[] = {&ctr, &n};
       int rc = launch(h, kinc_kernel_zero_i32(), 1, 64, args, 0, s);
       if ( rc ) return rc;
@@ -486,7 +535,7 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
    for ( int i = 0; i < kinc_sim::kAux; ++i ) HIP_TRY(h, hipStreamWaitEvent(h->aux[i], h->binned, 0));
    for ( int b = h->nbins - 1; b >= 0; --b )
    {
-      const int P = kSlotList[b];
+      const int P = h->classes[b];
       hipStream_t sb = h->aux[(h->nbins - 1 - b) % kinc_sim::kAux];
       const void* fn = kinc_select_pair_kernel(P, h->kmax);
       if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
@@ -494,10 +543,11 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
       HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
       if ( per_cu < 1 ) per_cu = 1;
       long long waves = (long long)per_cu * h->num_cus;
+      if ( P == 0 ) waves = std::min<long long>(waves, h->scratch_slots);
       long long grid = std::min<long long>(waves, (long long)cnt);
       a.list = h->d_lists + (size_t)b * (size_t)cnt;
       a.list_len = h->d_counters + b;
-      a.list_head = h->d_counters + kNumSlots + b;
+      a.list_head = h->d_counters + kMaxBins + b;
       void* args[] = {&a};
       int rc = launch_pairs(h, P, fn, (unsigned)grid, args, sb);
       if ( rc ) return rc;
@@ -552,6 +602,7 @@ extern "C" void kinc_sim_destroy(kinc_sim* h)
    (void)hipFree(h->d_counters);
    (void)hipFree(h->d_lists);
    (void)hipFree(h->d_binP);
+   (void)hipFree(h->d_scratch);
    (void)hipFree(h->d_K);
    (void)hipFree(h->d_labels);
    (void)hipFree(h->d_corr);

@@ -12,9 +12,12 @@
 #define KINC_TWO_LOG_2PI 3.6757541328186907
 
 /* Points held per lane in registers: a pair with N valid samples runs on the
- * kernel instance with 64*P >= N. */
+ * register-slot instance with 64*P >= N; above 64*KINC_MAX_P samples the
+ * memory-store instance (P = 0) runs. */
 #define KINC_MAX_P 16
-#define KINC_MAX_SAMPLES (64 * KINC_MAX_P)
+#define KINC_MAX_REG_SAMPLES (64 * KINC_MAX_P)
+/* Largest sample count (Spearman pads to the next power of two, 32768). */
+#define KINC_MAX_SAMPLES 32768
 
 /* Pair-kernel argument block (passed by value). */
 struct SimArgs
@@ -49,12 +52,22 @@ struct SimArgs
    float* corr_out;          /* [block_count][max_k] */
    uint16_t* iters_out;      /* [block_count][max_k] or null */
 
-   /* LDS carve-up (byte offsets in the dynamic shared segment) */
+   /* Working-set carve-up.  lds_prm and lds_tile are offsets in the dynamic
+    * shared segment.  lds_x, lds_lab, lds_w and lds_u are offsets in the
+    * shared segment too when scratch is null (register-slot instances), and
+    * offsets in the workgroup's scratch slice otherwise (memory-store
+    * instance, any sample count). */
    int32_t lds_x;            /* float2[S]  compact X, then the stale-read view */
    int32_t lds_lab;          /* int8[S]    full-index labels */
    int32_t lds_w;            /* int8[S]    work labels of the current fit */
    int32_t lds_prm;          /* float[8][8] component parameters */
-   int32_t lds_u;            /* union: chain-term tile / sort buffers */
+   int32_t lds_tile;         /* chain-term tile */
+   int32_t lds_u;            /* sort buffers / sequence scratch */
+   char* scratch;            /* memory-store instance: slices of scratch_stride bytes */
+   int64_t scratch_stride;
+   int32_t scr_lg;           /* float[(KMAX+1)][mem_slots][64] responsibilities, log p(x) */
+   int32_t scr_y;            /* int8[2][mem_slots][64] K-means labels */
+   int32_t mem_slots;        /* ceil(S / 64) */
    int32_t np2;              /* next power of two >= S (min 2) */
    int32_t tile_stride;      /* floats per chain row of the term tile */
 };

@@ -136,22 +136,48 @@ __device__ __forceinline__ void pair_xy(int64_t k, int& x, int& y)
 
 struct Lds
 {
-   float2* X;      /* [S] */
-   int8_t* lab;    /* [S] */
-   int8_t* w;      /* [S] */
-   float* prm;     /* [KMAX][8] component parameters / K-means means */
-   float* u;       /* union region */
+   float2* X;      /* [S] compact X, then the stale-read view */
+   int8_t* lab;    /* [S] full-index labels */
+   int8_t* w;      /* [S] work labels of the current fit */
+   float* prm;     /* [KMAX][8] component parameters / K-means means (LDS) */
+   float* tile;    /* chain-term tile (LDS) */
+   float* u;       /* sort buffers / sequence scratch */
+   float* lgbuf;   /* memory-store variant: per-point responsibilities */
+   int8_t* ybuf;   /* memory-store variant: per-point K-means labels */
 };
 
+/*
+ * Working set of the pair a wave is processing.  Register-slot instances
+ * keep everything in LDS (tile and sort buffers share one region); the
+ * memory-store instance (P = 0, any sample count) keeps the per-pair arrays
+ * in the wave's slice of a device scratch buffer and only the parameters and
+ * the chain tile in LDS.
+ */
 __device__ __forceinline__ Lds lds_view(const SimArgs& a)
 {
    extern __shared__ __align__(16) char lds_raw[];
    Lds l;
-   l.X = (float2*)(lds_raw + a.lds_x);
-   l.lab = (int8_t*)(lds_raw + a.lds_lab);
-   l.w = (int8_t*)(lds_raw + a.lds_w);
    l.prm = (float*)(lds_raw + a.lds_prm);
-   l.u = (float*)(lds_raw + a.lds_u);
+   l.tile = (float*)(lds_raw + a.lds_tile);
+   if ( a.scratch == nullptr )
+   {
+      l.X = (float2*)(lds_raw + a.lds_x);
+      l.lab = (int8_t*)(lds_raw + a.lds_lab);
+      l.w = (int8_t*)(lds_raw + a.lds_w);
+      l.u = (float*)(lds_raw + a.lds_u);
+      l.lgbuf = nullptr;
+      l.ybuf = nullptr;
+   }
+   else
+   {
+      char* base = a.scratch + (size_t)blockIdx.x * (size_t)a.scratch_stride;
+      l.X = (float2*)(base + a.lds_x);
+      l.lab = (int8_t*)(base + a.lds_lab);
+      l.w = (int8_t*)(base + a.lds_w);
+      l.u = (float*)(base + a.lds_u);
+      l.lgbuf = (float*)(base + a.scr_lg);
+      l.ybuf = (int8_t*)(base + a.scr_y);
+   }
    return l;
 }
 
@@ -336,6 +362,86 @@ __device__ __forceinline__ float chain_add(const float* tile, int stride, int nc
    return acc;
 }
 
+/* ---------------------------------------------------------- point stores */
+
+/*
+ * Per-point state of one fit.  RegStore keeps point i (lane i%64, slot i/64)
+ * in VGPRs for P compile-time slots; MemStore keeps it in the wave's scratch
+ * slice, [slot][lane] so that every access is one coalesced 256-byte wave
+ * access, for any number of slots.  Both expose the same accessors and the
+ * fits below are written once against them.
+ */
+template <int P, int K>
+struct RegStore
+{
+   static constexpr int PMAX = P;
+   float x0[P], x1[P];
+   float lg[K][P];
+   float lpx[P];
+   int8_t y[P], yn[P];
+
+   __device__ __forceinline__ void init(const Lds& L, int N)
+   {
+      const int lane = lane_id();
+#pragma unroll
+      for ( int m = 0; m < P; ++m )
+      {
+         const int i = lane + m * WAVE;
+         const float2 v = (i < N) ? L.X[i] : make_float2(0.0f, 0.0f);
+         x0[m] = v.x;
+         x1[m] = v.y;
+      }
+   }
+   __device__ __forceinline__ float X0(int m) const { return x0[m]; }
+   __device__ __forceinline__ float X1(int m) const { return x1[m]; }
+   __device__ __forceinline__ float& LG(int k, int m) { return lg[k][m]; }
+   __device__ __forceinline__ float& LPX(int m) { return lpx[m]; }
+   __device__ __forceinline__ int8_t& Y(int m) { return y[m]; }
+   __device__ __forceinline__ int8_t& YN(int m) { return yn[m]; }
+};
+
+template <int K>
+struct MemStore
+{
+   static constexpr int PMAX = 0;
+   const float2* X;
+   float* lgb;
+   int8_t* yb;
+   int ms;
+   int n;
+
+   __device__ __forceinline__ void init(const Lds& L, int N, int slots)
+   {
+      X = L.X;
+      lgb = L.lgbuf;
+      yb = L.ybuf;
+      ms = slots;
+      n = N;
+   }
+   __device__ __forceinline__ float X0(int m) const
+   {
+      const int i = lane_id() + m * WAVE;
+      return (i < n) ? X[i].x : 0.0f;
+   }
+   __device__ __forceinline__ float X1(int m) const
+   {
+      const int i = lane_id() + m * WAVE;
+      return (i < n) ? X[i].y : 0.0f;
+   }
+   __device__ __forceinline__ float& LG(int k, int m) { return lgb[((size_t)(k * ms + m) << 6) + lane_id()]; }
+   __device__ __forceinline__ float& LPX(int m) { return lgb[((size_t)(K * ms + m) << 6) + lane_id()]; }
+   __device__ __forceinline__ int8_t& Y(int m) { return yb[((size_t)m << 6) + lane_id()]; }
+   __device__ __forceinline__ int8_t& YN(int m) { return yb[((size_t)(ms + m) << 6) + lane_id()]; }
+};
+
+/* loop bound over slots: the compile-time slot count of a register store
+ * (fully unrolled), the runtime count otherwise */
+template <class ST>
+__device__ __forceinline__ constexpr int slot_bound(int nslots)
+{
+   return ST::PMAX > 0 ? ST::PMAX : nslots;
+}
+
 /* ------------------------------------------------------------------ GMM */
 
 struct FitOut
@@ -382,12 +488,11 @@ __device__ __forceinline__ float lane_pick(const float (&v)[K])
    return r;
 }
 
-/* E step for the slots in use: lg[k][m] = log N(x|k) - log p(x),
- * lpx[m] = log p(x) (pairwise_gmm.cpp:54-77,143-187).  Slots past the
+/* E step for the slots in use: LG(k,m) = log N(x|k) - log p(x),
+ * LPX(m) = log p(x) (pairwise_gmm.cpp:54-77,143-187).  Lanes past the
  * valid points compute harmless values that are never read. */
-template <int P, int K>
-__device__ __forceinline__ void gmm_estep(int nslots, const float (&x0)[P], const float (&x1)[P],
-                                          const float* prm, float (&lg)[K][P], float (&lpx)[P])
+template <class ST, int K>
+__device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm)
 {
    float logpi[K];
 #pragma unroll
@@ -397,38 +502,41 @@ __device__ __forceinline__ void gmm_estep(int nslots, const float (&x0)[P], cons
       const float4 B = *(const float4*)(prm + 8 * k + 4);
       logpi[k] = B.w;
 #pragma unroll
-      for ( int m = 0; m < P; ++m )
+      for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
       {
          if ( m < nslots )
          {
-            float xm0 = x0[m] - A.x;
-            float xm1 = x1[m] - A.y;
+            float xm0 = st.X0(m) - A.x;
+            float xm1 = st.X1(m) - A.y;
             float b0 = A.z * xm0 + A.w * xm1;
             float b1 = B.x * xm0 + B.y * xm1;
             float dot = xm0 * b0 + xm1 * b1;
-            lg[k][m] = B.z - 0.5f * dot;
+            st.LG(k, m) = B.z - 0.5f * dot;
          }
       }
    }
 #pragma unroll
-   for ( int m = 0; m < P; ++m )
+   for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
    {
       if ( m < nslots )
       {
+         float lgm[K];
+#pragma unroll
+         for ( int k = 0; k < K; ++k ) lgm[k] = st.LG(k, m);
          float mx = -INFINITY;
 #pragma unroll
          for ( int k = 0; k < K; ++k )
          {
-            float v = logpi[k] + lg[k][m];
+            float v = logpi[k] + lgm[k];
             mx = (v > mx) ? v : mx;
          }
          float sum = 0.0f;
 #pragma unroll
-         for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf((logpi[k] + lg[k][m]) - mx);
+         for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf((logpi[k] + lgm[k]) - mx);
          float lp = mx + kinc_logf(sum);
-         lpx[m] = lp;
+         st.LPX(m) = lp;
 #pragma unroll
-         for ( int k = 0; k < K; ++k ) lg[k][m] = lg[k][m] + (-lp);
+         for ( int k = 0; k < K; ++k ) st.LG(k, m) = lgm[k] + (-lp);
       }
    }
 }
@@ -436,14 +544,14 @@ __device__ __forceinline__ void gmm_estep(int nslots, const float (&x0)[P], cons
 /* GMM::fit (pairwise_gmm.cpp:367-445) with the per-fit LCG.  Writes the
  * fit's labels for i < N into L.w.  Per-component scalar work of the M step
  * runs with component k on lane k. */
-template <int P, int K>
-__device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
-                                          const float (&x0)[P], const float (&x1)[P], bool want_entropy)
+template <class ST, int K>
+__device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N, ST& st,
+                                          bool want_entropy)
 {
    const int lane = lane_id();
    const int STR = a.tile_stride;
    const int nslots = (N + WAVE - 1) / WAVE;
-   float* tile = L.u;
+   float* tile = L.tile;
    float* prm = L.prm;
    FitOut out;
    out.ok = true;
@@ -452,12 +560,12 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
 
    /* initialisation (pairwise_gmm.cpp:372-392): K draws of rand() % N */
    {
-      uint64_t st = 1;
+      uint64_t rs = 1;
       int my = 0;
 #pragma unroll
       for ( int k = 0; k < K; ++k )
       {
-         int i = lcg(st) % N;
+         int i = lcg(rs) % N;
          my = (lane == k) ? i : my;
       }
       if ( lane < K )
@@ -472,8 +580,6 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       wave_sync();
    }
 
-   float lg[K][P];
-   float lpx[P];
    float prevLogL = -INFINITY;
    float curLogL = -INFINITY;
    int e_steps = 0;
@@ -482,7 +588,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
 
    for ( int t = 0; t < 100; ++t )
    {
-      gmm_estep<P, K>(nslots, x0, x1, prm, lg, lpx);
+      gmm_estep<ST, K>(nslots, st, prm);
       e_steps++;
 
       float gmax[K];
@@ -491,12 +597,13 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       {
          float v = -INFINITY;
 #pragma unroll
-         for ( int m = 0; m < P; ++m )
+         for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
          {
             if ( m < nslots )
             {
                const bool valid = lane + m * WAVE < N;
-               v = (valid && lg[k][m] > v) ? lg[k][m] : v;
+               const float l = st.LG(k, m);
+               v = (valid && l > v) ? l : v;
             }
          }
          gmax[k] = wave_max(v);
@@ -505,19 +612,22 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       /* pass A: chain 0 = logL, 1+k = Gamma_k sum, 1+K+2k / 2+K+2k = mu_k sums */
       float acc = 0.0f;
 #pragma unroll
-      for ( int m = 0; m < P; ++m )
+      for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
       {
          if ( m < nslots )
          {
-            tile[lane] = lpx[m];
+            tile[lane] = st.LPX(m);
+            const float xa = st.X0(m);
+            const float xb = st.X1(m);
 #pragma unroll
             for ( int k = 0; k < K; ++k )
             {
-               tile[(1 + k) * STR + lane] = kinc_expf(lg[k][m] - gmax[k]);
-               const float gm = kinc_expf(lg[k][m]);
-               lg[k][m] = gm;
-               tile[(1 + K + 2 * k) * STR + lane] = gm * x0[m];
-               tile[(2 + K + 2 * k) * STR + lane] = gm * x1[m];
+               const float l = st.LG(k, m);
+               tile[(1 + k) * STR + lane] = kinc_expf(l - gmax[k]);
+               const float gm = kinc_expf(l);
+               st.LG(k, m) = gm;
+               tile[(1 + K + 2 * k) * STR + lane] = gm * xa;
+               tile[(2 + K + 2 * k) * STR + lane] = gm * xb;
             }
             wave_sync();
             int len = N - m * WAVE;
@@ -580,16 +690,18 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          muk1[k] = mu.y;
       }
 #pragma unroll
-      for ( int m = 0; m < P; ++m )
+      for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
       {
          if ( m < nslots )
          {
+            const float xa = st.X0(m);
+            const float xb = st.X1(m);
 #pragma unroll
             for ( int k = 0; k < K; ++k )
             {
-               const float xm0 = x0[m] - muk0[k];
-               const float xm1 = x1[m] - muk1[k];
-               const float gm = lg[k][m];
+               const float xm0 = xa - muk0[k];
+               const float xm1 = xb - muk1[k];
+               const float gm = st.LG(k, m);
                tile[(3 * k) * STR + lane] = gm * (xm0 * xm0);
                tile[(3 * k + 1) * STR + lane] = gm * (xm0 * xm1);
                tile[(3 * k + 2) * STR + lane] = gm * (xm1 * xm1);
@@ -634,16 +746,15 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
     * that ran 100 iterations labels from the exponentiated ones. */
    if ( converged )
    {
-      gmm_estep<P, K>(nslots, x0, x1, prm, lg, lpx);
+      gmm_estep<ST, K>(nslots, st, prm);
    }
 
-   /* calcLabels (pairwise_gmm.cpp:323-341) and the entropy terms (:348-360) */
-   float ent[P];
+   /* calcLabels (pairwise_gmm.cpp:323-341); the entropy terms (:348-360)
+    * go to LPX, which is free after the last E step */
 #pragma unroll
-   for ( int m = 0; m < P; ++m )
+   for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
    {
       const int i = lane + m * WAVE;
-      ent[m] = 0.0f;
       if ( m < nslots )
       {
          int max_k = -1;
@@ -651,14 +762,15 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
 #pragma unroll
          for ( int k = 0; k < K; ++k )
          {
-            if ( mg < lg[k][m] )
+            const float l = st.LG(k, m);
+            if ( mg < l )
             {
                max_k = k;
-               mg = lg[k][m];
+               mg = l;
             }
          }
          if ( i < N ) L.w[i] = (int8_t)max_k;
-         if ( want_entropy ) ent[m] = (max_k >= 0) ? kinc_logf(mg) : NAN;
+         if ( want_entropy ) st.LPX(m) = (max_k >= 0) ? kinc_logf(mg) : NAN;
       }
    }
    wave_sync();
@@ -667,11 +779,11 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
    {
       float accE = 0.0f;
 #pragma unroll
-      for ( int m = 0; m < P; ++m )
+      for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
       {
          if ( m < nslots )
          {
-            tile[lane] = ent[m];
+            tile[lane] = st.LPX(m);
             wave_sync();
             int len = N - m * WAVE;
             len = len < TILE ? len : TILE;
@@ -697,22 +809,20 @@ __device__ __forceinline__ float diff_norm(float ax, float ay, float bx, float b
 
 /* KMeans::fit (pairwise_kmeans.cpp:12-127) with the per-fit LCG.  Means live
  * in LDS (2 floats per cluster); the best init's labels go to L.w. */
-template <int P, int K>
-__device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N,
-                                             const float (&x0)[P], const float (&x1)[P])
+template <class ST, int K>
+__device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N, ST& st)
 {
    const int lane = lane_id();
    const int STR = a.tile_stride;
    const int nslots = (N + WAVE - 1) / WAVE;
-   float* tile = L.u;
+   float* tile = L.tile;
    float* means = L.prm;
    FitOut out;
    out.ok = true;
    out.E = 0.0f;
-   uint64_t st = 1;
+   uint64_t rs = 1;
    float bestLogL = -INFINITY;
    int iters = 0;
-   int8_t y[P];
 
    for ( int init = 0; init < 10; ++init )
    {
@@ -720,14 +830,17 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
 #pragma unroll
       for ( int k = 0; k < K; ++k )
       {
-         int i = lcg(st) % N;
+         int i = lcg(rs) % N;
          my = (lane == k) ? i : my;
       }
       wave_sync();
       if ( lane < K ) *(float2*)(means + 2 * lane) = L.X[my];
       wave_sync();
 #pragma unroll
-      for ( int m = 0; m < P; ++m ) y[m] = 0;
+      for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
+      {
+         if ( m < nslots ) st.Y(m) = 0;
+      }
 
       for ( int t = 0; t < 300; ++t )
       {
@@ -740,47 +853,53 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
             m0[k] = mk.x;
             m1[k] = mk.y;
          }
-         int8_t yn[P];
          bool diff = false;
 #pragma unroll
-         for ( int m = 0; m < P; ++m )
+         for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
          {
-            yn[m] = 0;
             if ( m < nslots )
             {
+               const float xa = st.X0(m);
+               const float xb = st.X1(m);
                int min_k = -1;
                float min_d = 0.0f;
 #pragma unroll
                for ( int k = 0; k < K; ++k )
                {
-                  const float d = diff_norm(x0[m], x1[m], m0[k], m1[k]);
+                  const float d = diff_norm(xa, xb, m0[k], m1[k]);
                   if ( min_k == -1 || d < min_d )
                   {
                      min_k = k;
                      min_d = d;
                   }
                }
-               yn[m] = (int8_t)min_k;
-               diff = diff || ((lane + m * WAVE < N) && yn[m] != y[m]);
+               st.YN(m) = (int8_t)min_k;
+               diff = diff || ((lane + m * WAVE < N) && (int8_t)min_k != st.Y(m));
             }
          }
          if ( ballot(diff) == 0 ) break;
 #pragma unroll
-         for ( int m = 0; m < P; ++m ) y[m] = yn[m];
+         for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
+         {
+            if ( m < nslots ) st.Y(m) = st.YN(m);
+         }
 
          /* means: chain 2k / 2k+1 over members in index order */
          float acc = 0.0f;
 #pragma unroll
-         for ( int m = 0; m < P; ++m )
+         for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
          {
             if ( m < nslots )
             {
+               const float xa = st.X0(m);
+               const float xb = st.X1(m);
+               const int ym = st.Y(m);
 #pragma unroll
                for ( int k = 0; k < K; ++k )
                {
-                  const bool mem = y[m] == k;
-                  tile[(2 * k) * STR + lane] = mem ? x0[m] : 0.0f;
-                  tile[(2 * k + 1) * STR + lane] = mem ? x1[m] : 0.0f;
+                  const bool mem = ym == k;
+                  tile[(2 * k) * STR + lane] = mem ? xa : 0.0f;
+                  tile[(2 * k + 1) * STR + lane] = mem ? xb : 0.0f;
                }
                wave_sync();
                int len = N - m * WAVE;
@@ -795,9 +914,9 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
          {
             int cnt = 0;
 #pragma unroll
-            for ( int m = 0; m < P; ++m )
+            for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
             {
-               if ( m < nslots ) cnt += popc(ballot(lane + m * WAVE < N && y[m] == k));
+               if ( m < nslots ) cnt += popc(ballot(lane + m * WAVE < N && st.Y(m) == k));
             }
             nk = (lane == k) ? cnt : nk;
          }
@@ -818,15 +937,15 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
          const float2 mk = *(const float2*)(means + 2 * k);
          int cnt = 0;
 #pragma unroll
-         for ( int m = 0; m < P; ++m )
+         for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
          {
             if ( m < nslots )
             {
-               const bool mem = (lane + m * WAVE < N) && y[m] == k;
+               const bool mem = (lane + m * WAVE < N) && st.Y(m) == k;
                const uint64_t bm = ballot(mem);
                if ( mem )
                {
-                  const float d = diff_norm(x0[m], x1[m], mk.x, mk.y);
+                  const float d = diff_norm(st.X0(m), st.X1(m), mk.x, mk.y);
                   seq[kbase + cnt + popc(bm & lanes_below())] = d * d;
                }
                cnt += popc(bm);
@@ -847,10 +966,10 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
       {
          bestLogL = logL;
 #pragma unroll
-         for ( int m = 0; m < P; ++m )
+         for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
          {
             const int i = lane + m * WAVE;
-            if ( m < nslots && i < N ) L.w[i] = y[m];
+            if ( m < nslots && i < N ) L.w[i] = st.Y(m);
          }
          wave_sync();
       }
@@ -1050,35 +1169,46 @@ __device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
 
 /* -------------------------------------------------------------- driver */
 
-/* One clustering fit with Kc components, Kc <= KMAX (compile-time K). */
+/* One clustering fit with K components; P > 0 selects the register store
+ * with P slots, P = 0 the memory store. */
 template <int P, int K>
-__device__ __forceinline__ FitOut fit_one(const SimArgs& a, const Lds& L, int N,
-                                          const float (&x0)[P], const float (&x1)[P])
+__device__ __forceinline__ FitOut fit_one(const SimArgs& a, const Lds& L, int N)
 {
-   return (a.clus == 1) ? gmm_fit<P, K>(a, L, N, x0, x1, a.crit == 1)
-                        : kmeans_fit<P, K>(a, L, N, x0, x1);
+   if constexpr ( P > 0 )
+   {
+      RegStore<P, K> st;
+      st.init(L, N);
+      return (a.clus == 1) ? gmm_fit<RegStore<P, K>, K>(a, L, N, st, a.crit == 1)
+                           : kmeans_fit<RegStore<P, K>, K>(a, L, N, st);
+   }
+   else
+   {
+      MemStore<K> st;
+      st.init(L, N, a.mem_slots);
+      return (a.clus == 1) ? gmm_fit<MemStore<K>, K>(a, L, N, st, a.crit == 1)
+                           : kmeans_fit<MemStore<K>, K>(a, L, N, st);
+   }
 }
 
 template <int P, int KMAX>
-__device__ __forceinline__ FitOut fit_dispatch(const SimArgs& a, const Lds& L, int N, int Kc,
-                                               const float (&x0)[P], const float (&x1)[P])
+__device__ __forceinline__ FitOut fit_dispatch(const SimArgs& a, const Lds& L, int N, int Kc)
 {
    switch ( Kc )
    {
-   case 1: return fit_one<P, 1>(a, L, N, x0, x1);
-   case 2: return fit_one<P, 2>(a, L, N, x0, x1);
-   case 3: return fit_one<P, 3>(a, L, N, x0, x1);
-   case 4: return fit_one<P, 4>(a, L, N, x0, x1);
-   case 5: return fit_one<P, 5>(a, L, N, x0, x1);
+   case 1: return fit_one<P, 1>(a, L, N);
+   case 2: return fit_one<P, 2>(a, L, N);
+   case 3: return fit_one<P, 3>(a, L, N);
+   case 4: return fit_one<P, 4>(a, L, N);
+   case 5: return fit_one<P, 5>(a, L, N);
    default: break;
    }
-   if ( KMAX > 5 )
+   if constexpr ( KMAX > 5 )
    {
       switch ( Kc )
       {
-      case 6: return fit_one<P, (KMAX > 5 ? 6 : 1)>(a, L, N, x0, x1);
-      case 7: return fit_one<P, (KMAX > 5 ? 7 : 1)>(a, L, N, x0, x1);
-      case 8: return fit_one<P, (KMAX > 5 ? 8 : 1)>(a, L, N, x0, x1);
+      case 6: return fit_one<P, 6>(a, L, N);
+      case 7: return fit_one<P, 7>(a, L, N);
+      case 8: return fit_one<P, 8>(a, L, N);
       default: break;
       }
    }
@@ -1116,19 +1246,10 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
       int bestK = 0;
       if ( N >= a.min_samples )
       {
-         float x0[P], x1[P];
-#pragma unroll
-         for ( int m = 0; m < P; ++m )
-         {
-            int i = lane + m * WAVE;
-            float2 v = (i < N) ? L.X[i] : make_float2(0.0f, 0.0f);
-            x0[m] = v.x;
-            x1[m] = v.y;
-         }
          float bestValue = INFINITY;
          for ( int Kc = a.min_k; Kc <= a.max_k; ++Kc )
          {
-            FitOut f = fit_dispatch<P, KMAX>(a, L, N, Kc, x0, x1);
+            FitOut f = fit_dispatch<P, KMAX>(a, L, N, Kc);
             if ( lane == Kc - 1 ) itv = f.iters;
             if ( !f.ok ) continue;
 
@@ -1207,8 +1328,13 @@ __global__ KINC_PAIR_BOUNDS void k_pairs(SimArgs a)
 {
    if ( a.list == nullptr )
    {
-      const int q = a.first + (int)blockIdx.x;
-      if ( q < a.first + a.launch_count ) process_pair<P, KMAX>(a, q);
+      /* the memory-store instance owns one scratch slice per workgroup, so
+       * its grid is capped and strides over the launch */
+      for ( int q = a.first + (int)blockIdx.x; q < a.first + a.launch_count; q += (int)gridDim.x )
+      {
+         process_pair<P, KMAX>(a, q);
+         if ( P > 0 ) break;
+      }
       return;
    }
    const int len = *a.list_len;

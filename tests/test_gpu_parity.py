@@ -31,13 +31,32 @@ CASES = [
      dict(clusmethod="kmeans", corrmethod="spearman", crit="BIC", minexpr=7.0), 0, 150),
     ("gmm_maxclus8_small_S", dict(kind="modules", G=40, S=64, seed=18, nan_rate=0.05),
      dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", maxclus=8, minclus=2), 0, 200),
+    # above 1024 samples: the memory-store instance
+    ("gmm_spearman_S1500", dict(kind="modules", G=14, S=1500, seed=19, nan_rate=0.10),
+     dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True), 0, 91),
+    ("kmeans_pearson_S1300", dict(kind="modules", G=12, S=1300, seed=20, nan_rate=0.15),
+     dict(clusmethod="kmeans", corrmethod="pearson", crit="BIC"), 0, 66),
+    ("gmm_spearman_E_S5000", dict(kind="modules", G=10, S=5000, seed=6, nan_rate=0.10,
+                                  degenerate=0.2),
+     dict(clusmethod="gmm", corrmethod="spearman", crit="BIC"), 0, 45),
 ]
 
 
 def _gem(kind, G, S, seed, **kw):
     if kind == "latent":
         return gemgen.latent(G, S, seed, kw.get("nan_rate", 0.05))
-    return gemgen.modules(G, S, seed, nan_rate=kw.get("nan_rate", 0.1))
+    return gemgen.modules(G, S, seed, nan_rate=kw.get("nan_rate", 0.1),
+                          degenerate=kw.get("degenerate", 0.0))
+
+
+_ORACLE = {}
+
+
+def _oracle_block(name, p, x, start, count):
+    """Oracle output of a case, computed once for both execution forms."""
+    if name not in _ORACLE:
+        _ORACLE[name] = oracle.execute_block(p, x, start, count)
+    return _ORACLE[name]
 
 
 @pytest.mark.parametrize("form", ["plain", "persistent"])
@@ -48,7 +67,7 @@ def test_block_parity(case, form):
     G, S = x.shape
     count = min(count, _native.pair_count(G) - start)
     p = _native.make_params(form=form, ksize=1000, **flags)
-    want = oracle.execute_block(p, x, start, count)
+    want = _oracle_block(name, p, x, start, count)
     eng = _native.Engine(x, p)
     try:
         got = eng.execute(start, count)
