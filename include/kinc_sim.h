@@ -76,7 +76,8 @@ typedef struct kinc_sim_params
    float   max_corr;       /* --maxcorr, default 1.0 (used by the CCM/CMX writer) */
    int32_t ksize;          /* --ksize, pairs per plain-form launch, default 4096 */
    int32_t form;           /* KINC_FORM_*, default persistent */
-   int32_t reserved[5];    /* must be zero */
+   int32_t batch_capacity; /* largest count of one execute call; 0 = 32768 */
+   int32_t reserved[4];    /* must be zero */
 } kinc_sim_params;
 
 typedef struct kinc_sim kinc_sim;
@@ -109,7 +110,8 @@ int kinc_sim_create(const kinc_sim_params* params, const float* emx,
  * A block is the unit of the reference's stale-buffer semantics: the X
  * workspace is shared by the pairs of one call in index order
  * (src/core/similarity_serial.cpp:42), so results depend on the block split
- * exactly as they do in the reference.  count <= KINC_WORK_BLOCK_SIZE.
+ * exactly as they do in the reference.  count <= KINC_WORK_BLOCK_SIZE (and
+ * <= batch_capacity).
  */
 int kinc_sim_execute(kinc_sim* h, int64_t start, int64_t count,
                      int8_t* K, int8_t* labels, float* corr, uint16_t* iters);
@@ -126,6 +128,25 @@ int kinc_sim_execute(kinc_sim* h, int64_t start, int64_t count,
 int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count,
                             int8_t* K, int8_t* labels, float* corr,
                             uint16_t* iters, void* stream);
+
+/*
+ * Execute an explicit list of pairs: pairs[2*q], pairs[2*q+1] are the genes
+ * (x, y) of pair q, x > y.  seg_lo[q] (optional, NULL = all 0) is the index of
+ * the first pair of q's work block within this call: the pairs of one work
+ * block share the reference's X workspace (stale-read Pearson semantics, see
+ * kinc_sim_execute), blocks do not.  Several requests' work blocks can so be
+ * batched into one call with results identical to running each block alone.
+ * count <= batch_capacity.  Host buffers in, host buffers out.
+ */
+int kinc_sim_execute_pairs(kinc_sim* h, const int32_t* pairs, const int32_t* seg_lo,
+                           int64_t count, int8_t* K, int8_t* labels, float* corr,
+                           uint16_t* iters);
+
+/* Same with device buffers (pairs as int32 x,y interleaved), asynchronous on
+ * `stream` like kinc_sim_execute_device. */
+int kinc_sim_execute_pairs_device(kinc_sim* h, const int32_t* pairs, const int32_t* seg_lo,
+                                  int64_t count, int8_t* K, int8_t* labels, float* corr,
+                                  uint16_t* iters, void* stream);
 
 /* Wait for all work queued by the handle. */
 int kinc_sim_synchronize(kinc_sim* h);

@@ -43,7 +43,8 @@ class SimParams(ctypes.Structure):
         ("max_corr", ctypes.c_float),
         ("ksize", ctypes.c_int32),
         ("form", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("batch_capacity", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -58,6 +59,9 @@ SIGNATURES = [
     ("kinc_sim_execute", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P]),
     ("kinc_sim_execute_device", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P, _P,
                                                _P, _P, _P]),
+    ("kinc_sim_execute_pairs", ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P, _P, _P, _P]),
+    ("kinc_sim_execute_pairs_device", ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P, _P, _P,
+                                                     _P, _P]),
     ("kinc_sim_synchronize", ctypes.c_int, [_P]),
     ("kinc_sim_destroy", None, [_P]),
     ("kinc_sim_last_error", ctypes.c_char_p, [_P]),
@@ -93,7 +97,7 @@ def default_params():
 
 def make_params(clusmethod="none", corrmethod="pearson", minexpr=float("-inf"), minsamp=30,
                 minclus=1, maxclus=5, crit="ICL", preout=False, postout=False, mincorr=0.5,
-                maxcorr=1.0, ksize=4096, form="persistent"):
+                maxcorr=1.0, ksize=4096, form="persistent", batch_capacity=0):
     """Build kinc_sim_params from the reference's CLI flag names and defaults
     (src/core/similarity_input.cpp:96-254)."""
     p = default_params()
@@ -110,6 +114,7 @@ def make_params(clusmethod="none", corrmethod="pearson", minexpr=float("-inf"), 
     p.max_corr = maxcorr
     p.ksize = ksize
     p.form = FORM[form]
+    p.batch_capacity = batch_capacity
     return p
 
 
@@ -160,6 +165,26 @@ class Engine:
                                     labels.ctypes.data if labels is not None else None,
                                     corr.ctypes.data,
                                     iters.ctypes.data if iters is not None else None)
+        self._check(rc)
+        return K, labels, corr, iters
+
+    def execute_pairs(self, pairs, seg_lo=None, want_labels=True, want_iters=True):
+        """Run an explicit pair list ((count, 2) int32 of x > y); seg_lo gives
+        each pair's work-block start within the list (default: one block)."""
+        pairs = np.ascontiguousarray(pairs, np.int32)
+        count = pairs.shape[0]
+        MC = self.params.max_clusters
+        K = np.empty(count, np.int8)
+        corr = np.empty((count, MC), np.float32)
+        labels = np.empty((count, self.S), np.int8) if want_labels else None
+        iters = np.empty((count, MC), np.uint16) if want_iters else None
+        seg = None if seg_lo is None else np.ascontiguousarray(seg_lo, np.int32)
+        rc = lib().kinc_sim_execute_pairs(self.h, pairs.ctypes.data,
+                                          seg.ctypes.data if seg is not None else None, count,
+                                          K.ctypes.data,
+                                          labels.ctypes.data if labels is not None else None,
+                                          corr.ctypes.data,
+                                          iters.ctypes.data if iters is not None else None)
         self._check(rc)
         return K, labels, corr, iters
 

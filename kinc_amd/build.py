@@ -34,6 +34,7 @@ SOURCES = [
     "sim_util_kernels.hip",
     "kinc_sim.cpp",
     "formats.cpp",
+    "server.cpp",
 ]
 # pair-kernel instances (KMAX, P): register-slot counts P, and P = 0 for the
 # memory-store instance used above 1024 samples

@@ -111,6 +111,9 @@ struct kinc_sim
    int nbins = 0;
    char* d_scratch = nullptr;      /* memory-store slices */
    int scratch_slots = 0;
+   int cap = KINC_WORK_BLOCK_SIZE; /* largest count of one call */
+   int32_t* d_pairs = nullptr;     /* staging of host pair lists [cap][2] */
+   int32_t* d_seg = nullptr;       /* staging of host segment starts [cap] */
    /* staging for the host-buffer entry point */
    int8_t* d_K = nullptr;
    int8_t* d_labels = nullptr;
@@ -198,6 +201,11 @@ static int validate(const kinc_sim_params* p, int G, int S, std::string& why)
       return KINC_E_INVALID;
    }
    if ( p->ksize < 1 ) { why = "ksize must be >= 1"; return KINC_E_INVALID; }
+   if ( p->batch_capacity < 0 || p->batch_capacity > (1 << 24) )
+   {
+      why = "batch_capacity must lie in [0, 2^24]";
+      return KINC_E_INVALID;
+   }
    if ( p->form != KINC_FORM_PLAIN && p->form != KINC_FORM_PERSISTENT ) { why = "form must be 0 (plain) or 1 (persistent)"; return KINC_E_INVALID; }
    for ( int r : p->reserved )
    {
@@ -272,7 +280,8 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    CREATE_TRY(hipMalloc(&h->d_logN, sizeof(double) * logN.size()));
    CREATE_TRY(hipMemcpy(h->d_logN, logN.data(), sizeof(double) * logN.size(), hipMemcpyHostToDevice));
 
-   const size_t B = KINC_WORK_BLOCK_SIZE;
+   h->cap = params->batch_capacity > 0 ? params->batch_capacity : KINC_WORK_BLOCK_SIZE;
+   const size_t B = (size_t)h->cap;
    CREATE_TRY(hipMalloc(&h->d_blockN, sizeof(int32_t) * B));
    CREATE_TRY(hipMalloc(&h->d_chunkMax, sizeof(int32_t) * (B / 64 + 1)));
    CREATE_TRY(hipMalloc(&h->d_counters, sizeof(int32_t) * 2 * kMaxBins));
@@ -359,7 +368,9 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
 static int ensure_staging(kinc_sim* h)
 {
    if ( h->d_K ) return KINC_OK;
-   const size_t B = KINC_WORK_BLOCK_SIZE;
+   const size_t B = (size_t)h->cap;
+   HIP_TRY(h, hipMalloc(&h->d_pairs, sizeof(int32_t) * 2 * B));
+   HIP_TRY(h, hipMalloc(&h->d_seg, sizeof(int32_t) * B));
    HIP_TRY(h, hipMalloc(&h->d_K, B));
    HIP_TRY(h, hipMalloc(&h->d_labels, B * (size_t)h->S));
    HIP_TRY(h, hipMalloc(&h->d_corr, sizeof(float) * B * (size_t)h->p.max_clusters));
@@ -448,22 +459,16 @@ extern "C" int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launc
    return n;
 }
 
-extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count,
-                                       int8_t* K, int8_t* labels, float* corr,
-                                       uint16_t* iters, void* stream)
+/* One call: `count` pairs given by a linear start (pairs == null) or by an
+ * explicit device pair list with optional work-block segment starts. */
+static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* seg_lo,
+               int cnt, int8_t* K, int8_t* labels, float* corr, uint16_t* iters, hipStream_t s)
 {
-   if ( !h ) return KINC_E_INVALID;
-   if ( count < 0 || count > KINC_WORK_BLOCK_SIZE ) return fail(h, KINC_E_INVALID, "count must lie in [0, 32768]");
-   if ( start < 0 || start + count > kinc_pair_count(h->G) ) return fail(h, KINC_E_INVALID, "pair range outside the matrix");
-   if ( count == 0 ) return KINC_OK;
-   if ( !K || !corr ) return fail(h, KINC_E_INVALID, "K and corr outputs are required");
-   HIP_TRY(h, hipSetDevice(h->device));
-   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-   const int cnt = (int)count;
-
    SimArgs a = h->base;
    a.block_start = start;
    a.block_count = cnt;
+   a.pairs = (const int2*)pairs;
+   a.seg_lo = seg_lo;
    a.K_out = K;
    a.labels_out = labels;
    a.corr_out = corr;
@@ -475,9 +480,11 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
       const float* emx = h->d_emx;
       int S = h->S;
       int64_t bs = start;
+      const int2* pl = a.pairs;
       float me = h->p.min_expr;
       int32_t* outN = h->d_blockN;
-      void* args[] = {&emx, &S, &bs, (void*)&cnt, &me, &outN};
+      int G = h->G;
+      void* args[] = {&emx, &S, &G, &bs, &pl, (void*)&cnt, &me, &outN};
       int rc = launch(h, kinc_kernel_count_valid(), (unsigned)((cnt + 3) / 4), 256, args, 0, s);
       if ( rc ) return rc;
    }
@@ -560,6 +567,68 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
    return KINC_OK;
 }
 
+extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count,
+                                       int8_t* K, int8_t* labels, float* corr,
+                                       uint16_t* iters, void* stream)
+{
+   if ( !h ) return KINC_E_INVALID;
+   if ( count < 0 || count > KINC_WORK_BLOCK_SIZE || count > h->cap )
+   {
+      return fail(h, KINC_E_INVALID, "count must lie in [0, min(32768, batch_capacity)]");
+   }
+   if ( start < 0 || start + count > kinc_pair_count(h->G) ) return fail(h, KINC_E_INVALID, "pair range outside the matrix");
+   if ( count == 0 ) return KINC_OK;
+   if ( !K || !corr ) return fail(h, KINC_E_INVALID, "K and corr outputs are required");
+   HIP_TRY(h, hipSetDevice(h->device));
+   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+   return run(h, start, nullptr, nullptr, (int)count, K, labels, corr, iters, s);
+}
+
+extern "C" int kinc_sim_execute_pairs_device(kinc_sim* h, const int32_t* pairs, const int32_t* seg_lo,
+                                             int64_t count, int8_t* K, int8_t* labels, float* corr,
+                                             uint16_t* iters, void* stream)
+{
+   if ( !h ) return KINC_E_INVALID;
+   if ( count < 0 || count > h->cap ) return fail(h, KINC_E_INVALID, "count must lie in [0, batch_capacity]");
+   if ( count == 0 ) return KINC_OK;
+   if ( !pairs || !K || !corr ) return fail(h, KINC_E_INVALID, "pairs, K and corr are required");
+   HIP_TRY(h, hipSetDevice(h->device));
+   hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+   return run(h, 0, pairs, seg_lo, (int)count, K, labels, corr, iters, s);
+}
+
+extern "C" int kinc_sim_execute_pairs(kinc_sim* h, const int32_t* pairs, const int32_t* seg_lo,
+                                      int64_t count, int8_t* K, int8_t* labels, float* corr,
+                                      uint16_t* iters)
+{
+   if ( !h ) return KINC_E_INVALID;
+   if ( count < 0 || count > h->cap ) return fail(h, KINC_E_INVALID, "count must lie in [0, batch_capacity]");
+   if ( count == 0 ) return KINC_OK;
+   if ( !pairs || !K || !corr ) return fail(h, KINC_E_INVALID, "pairs, K and corr are required");
+   for ( int64_t q = 0; q < count; ++q )
+   {
+      const int32_t x = pairs[2 * q], y = pairs[2 * q + 1];
+      if ( !(0 <= y && y < x && x < h->G) ) return fail(h, KINC_E_INVALID, "pair " + std::to_string(q) + " is not x > y >= 0 within the matrix");
+      if ( seg_lo && (seg_lo[q] < 0 || seg_lo[q] > q) ) return fail(h, KINC_E_INVALID, "seg_lo[q] must lie in [0, q]");
+   }
+   int rc = ensure_staging(h);
+   if ( rc ) return rc;
+   HIP_TRY(h, hipSetDevice(h->device));
+   HIP_TRY(h, hipMemcpyAsync(h->d_pairs, pairs, sizeof(int32_t) * 2 * (size_t)count, hipMemcpyHostToDevice, h->stream));
+   if ( seg_lo ) HIP_TRY(h, hipMemcpyAsync(h->d_seg, seg_lo, sizeof(int32_t) * (size_t)count, hipMemcpyHostToDevice, h->stream));
+   rc = run(h, 0, h->d_pairs, seg_lo ? h->d_seg : nullptr, (int)count, h->d_K, labels ? h->d_labels : nullptr,
+            h->d_corr, iters ? h->d_iters : nullptr, h->stream);
+   if ( rc ) return rc;
+   const size_t n = (size_t)count;
+   const size_t MC = (size_t)h->p.max_clusters;
+   HIP_TRY(h, hipMemcpyAsync(K, h->d_K, n, hipMemcpyDeviceToHost, h->stream));
+   HIP_TRY(h, hipMemcpyAsync(corr, h->d_corr, sizeof(float) * n * MC, hipMemcpyDeviceToHost, h->stream));
+   if ( labels ) HIP_TRY(h, hipMemcpyAsync(labels, h->d_labels, n * (size_t)h->S, hipMemcpyDeviceToHost, h->stream));
+   if ( iters ) HIP_TRY(h, hipMemcpyAsync(iters, h->d_iters, sizeof(uint16_t) * n * MC, hipMemcpyDeviceToHost, h->stream));
+   HIP_TRY(h, hipStreamSynchronize(h->stream));
+   return KINC_OK;
+}
+
 extern "C" int kinc_sim_execute(kinc_sim* h, int64_t start, int64_t count,
                                 int8_t* K, int8_t* labels, float* corr, uint16_t* iters)
 {
@@ -603,6 +672,8 @@ extern "C" void kinc_sim_destroy(kinc_sim* h)
    (void)hipFree(h->d_lists);
    (void)hipFree(h->d_binP);
    (void)hipFree(h->d_scratch);
+   (void)hipFree(h->d_pairs);
+   (void)hipFree(h->d_seg);
    (void)hipFree(h->d_K);
    (void)hipFree(h->d_labels);
    (void)hipFree(h->d_corr);

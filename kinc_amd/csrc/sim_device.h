@@ -43,6 +43,10 @@ struct SimArgs
    int32_t pre_out;
    int32_t post_out;
 
+   const int2* pairs;        /* explicit (x, y) of each block-relative pair, or null
+                                for linear indices from block_start */
+   const int32_t* seg_lo;    /* first block-relative pair of each pair's stale-read
+                                segment (the work block it belongs to), or null (0) */
    const double* logN;       /* log((double)n) for n in [0, S] */
    const int32_t* blockN;    /* valid-sample count of every pair of the block */
    const int32_t* chunkMax;  /* max of blockN over 64-pair chunks */

@@ -981,22 +981,43 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
 
 /* ------------------------------------------------- stale X (Pearson) */
 
-/* Last block-relative pair d < qp with blockN[d] > v, or -1. */
-__device__ int find_prev_greater(const SimArgs& a, int qp, int v)
+/* (x, y) of block-relative pair q */
+__device__ __forceinline__ void pair_genes(const SimArgs& a, int q, int& x, int& y)
+{
+   if ( a.pairs )
+   {
+      /* a pair outside x > y >= 0, x < G reads gene 0 for both rows so that
+       * no access leaves the matrix; its results are meaningless, and the
+       * host entry point rejects such pairs before launching */
+      const int2 v = a.pairs[q];
+      const bool ok = 0 <= v.y && v.y < v.x && v.x < a.G;
+      x = ok ? v.x : 0;
+      y = ok ? v.y : 0;
+   }
+   else
+   {
+      pair_xy(a.block_start + q, x, y);
+   }
+}
+
+/* Last block-relative pair d with lo <= d < qp and blockN[d] > v, or -1.
+ * Chunks of 64 pairs lying wholly in [lo, qp) are skipped by their maxima. */
+__device__ int find_prev_greater(const SimArgs& a, int qp, int v, int lo)
 {
    const int lane = lane_id();
-   if ( qp <= 0 ) return -1;
+   if ( qp <= lo ) return -1;
    const int c = (qp - 1) >> 6;
    {
       int idx = c * WAVE + lane;
-      bool ok = idx < qp && a.blockN[idx] > v;
+      bool ok = idx >= lo && idx < qp && a.blockN[idx] > v;
       uint64_t m = ballot(ok);
       if ( m ) return c * WAVE + 63 - __clzll(m);
    }
-   for ( int cb = c - 1; cb >= 0; cb -= WAVE )
+   const int cfull = (lo + WAVE - 1) >> 6;   /* first chunk wholly >= lo */
+   for ( int cb = c - 1; cb >= cfull; cb -= WAVE )
    {
       int cc = cb - lane;
-      bool ok = cc >= 0 && a.chunkMax[cc] > v;
+      bool ok = cc >= cfull && a.chunkMax[cc] > v;
       uint64_t m = ballot(ok);
       if ( m )
       {
@@ -1005,6 +1026,14 @@ __device__ int find_prev_greater(const SimArgs& a, int qp, int v)
          uint64_t m2 = ballot(a.blockN[idx] > v);
          return ch * WAVE + 63 - __clzll(m2);
       }
+   }
+   const int cpart = lo >> 6;                /* chunk holding lo, if partial */
+   if ( cpart < cfull && cpart < c )
+   {
+      int idx = cpart * WAVE + lane;
+      bool ok = idx >= lo && a.blockN[idx] > v;
+      uint64_t m = ballot(ok);
+      if ( m ) return cpart * WAVE + 63 - __clzll(m);
    }
    return -1;
 }
@@ -1020,11 +1049,12 @@ __device__ void fill_stale(const SimArgs& a, const Lds& L, int q, int N)
 {
    const int S = a.S;
    const int lane = lane_id();
+   const int lo = a.seg_lo ? a.seg_lo[q] : 0;
    int v = N;
    int qp = q;
    while ( v < S )
    {
-      int d = find_prev_greater(a, qp, v);
+      int d = find_prev_greater(a, qp, v, lo);
       if ( d < 0 )
       {
          for ( int i = v + lane; i < S; i += WAVE ) L.X[i] = make_float2(0.0f, 0.0f);
@@ -1032,7 +1062,7 @@ __device__ void fill_stale(const SimArgs& a, const Lds& L, int q, int N)
       }
       const int hi = a.blockN[d];
       int gx, gy;
-      pair_xy(a.block_start + d, gx, gy);
+      pair_genes(a, d, gx, gy);
       const float* r1 = a.emx + (size_t)gx * (size_t)S;
       const float* r2 = a.emx + (size_t)gy * (size_t)S;
       int base = 0;
@@ -1229,7 +1259,7 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
    const int MC = a.max_k;
 
    int gx, gy;
-   pair_xy(a.block_start + q, gx, gy);
+   pair_genes(a, q, gx, gy);
    const int N = fetch_pair(a, gx, gy, L);
 
    /* iteration record of the fit with k+1 clusters lives in lane k */

@@ -6,13 +6,24 @@
 #include "sim_pair_impl.h"
 
 /* Valid-sample count per pair of a block (one wave per pair, 4 per group). */
-__global__ __launch_bounds__(256) void k_count_valid(const float* emx, int S, int64_t block_start,
+__global__ __launch_bounds__(256) void k_count_valid(const float* emx, int S, int G,
+                                                     int64_t block_start, const int2* pairs,
                                                      int count, float min_expr, int32_t* outN)
 {
    const int w = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
    if ( w >= count ) return;
    int gx, gy;
-   pair_xy(block_start + w, gx, gy);
+   if ( pairs )
+   {
+      const int2 v = pairs[w];
+      const bool ok = 0 <= v.y && v.y < v.x && v.x < G;
+      gx = ok ? v.x : 0;
+      gy = ok ? v.y : 0;
+   }
+   else
+   {
+      pair_xy(block_start + w, gx, gy);
+   }
    const float* r1 = emx + (size_t)gx * (size_t)S;
    const float* r2 = emx + (size_t)gy * (size_t)S;
    int n = 0;

@@ -45,6 +45,7 @@ def lib():
             "oracle_execute_block": (ctypes.c_int, [P, P, i32, i32, i64, i64, P, P, P, P]),
             "oracle_corr_fp64_block": (ctypes.c_int, [P, P, i32, i32, i64, i64, P, P, P]),
             "oracle_math_array": (None, [ctypes.c_int, P, P, i64]),
+            "oracle_execute_pairs": (ctypes.c_int, [P, P, i32, i32, P, P, i64, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -167,3 +168,21 @@ def math_array(fn, x):
     out = np.empty_like(x)
     lib().oracle_math_array(fn, x.ctypes.data, out.ctypes.data, x.size)
     return out
+
+
+def execute_pairs(params, emx, pairs, seg_lo=None, want_labels=True):
+    """Serial restatement over an explicit pair list; seg_lo marks work blocks."""
+    emx = np.ascontiguousarray(emx, np.float32)
+    G, S = emx.shape
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    count = pairs.shape[0]
+    MC = params.max_clusters
+    K = np.empty(count, np.int8)
+    labels = np.empty((count, S), np.int8) if want_labels else None
+    corr = np.empty((count, MC), np.float32)
+    iters = np.empty((count, MC), np.uint16)
+    seg = None if seg_lo is None else np.ascontiguousarray(seg_lo, np.int32)
+    lib().oracle_execute_pairs(ctypes.byref(params), emx.ctypes.data, G, S, pairs.ctypes.data,
+                               _ptr(seg), count, K.ctypes.data, _ptr(labels), corr.ctypes.data,
+                               iters.ctypes.data)
+    return K, labels, corr, iters

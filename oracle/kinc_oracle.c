@@ -802,6 +802,55 @@ int oracle_execute_block(const kinc_sim_params* p, const float* emx,
    return KINC_OK;
 }
 
+/*
+ * An explicit pair list: pairs[2q], pairs[2q+1] = (x, y).  seg_lo[q] (NULL =
+ * all 0) is the first pair of q's work block: every block starts with a
+ * fresh, zeroed X workspace, exactly as separate Serial::execute calls do.
+ */
+int oracle_execute_pairs(const kinc_sim_params* p, const float* emx, int32_t G, int32_t S,
+                         const int32_t* pairs, const int32_t* seg_lo, int64_t count,
+                         int8_t* K_out, int8_t* labels_out, float* corr_out, uint16_t* iters_out)
+{
+   const int MC = p->max_clusters;
+   const int NP2 = next_power2(S);
+   vec2* X = (vec2*)calloc((size_t)S, sizeof(vec2));
+   int8_t* labels = (int8_t*)malloc((size_t)S);
+   int8_t* wlabels = (int8_t*)calloc((size_t)S, 1);
+   float* work = (float*)malloc(sizeof(float) * (size_t)S);
+   float* wx = (float*)malloc(sizeof(float) * (size_t)NP2);
+   float* wy = (float*)malloc(sizeof(float) * (size_t)NP2);
+   float* wr = (float*)malloc(sizeof(float) * (size_t)NP2);
+   (void)G;
+   for ( int64_t q = 0; q < count; ++q )
+   {
+      const int32_t lo = seg_lo ? seg_lo[q] : 0;
+      if ( q == lo ) memset(X, 0, sizeof(vec2) * (size_t)S);   /* a new work block */
+      const float* g1 = emx + (size_t)pairs[2 * q] * (size_t)S;
+      const float* g2 = emx + (size_t)pairs[2 * q + 1] * (size_t)S;
+      int N = fetch_pair(g1, g2, S, p->min_expr, X, labels);
+      uint16_t* it = iters_out ? iters_out + q * MC : NULL;
+      if ( it ) memset(it, 0, sizeof(uint16_t) * (size_t)MC);
+      int K = 1;
+      if ( p->clus_method != KINC_CLUS_NONE )
+      {
+         K = clustering_compute(p, X, N, labels, wlabels, work, it);
+      }
+      float* corr = corr_out + q * MC;
+      for ( int k = 0; k < MC; ++k ) corr[k] = NAN;
+      for ( int k = 0; k < K; ++k )
+      {
+         corr[k] = (p->corr_method == KINC_CORR_PEARSON)
+            ? oracle_pearson(X, labels, S, (int8_t)k, p->min_samples)
+            : oracle_spearman(X, labels, S, (int8_t)k, p->min_samples, wx, wy, wr);
+      }
+      K_out[q] = (int8_t)K;
+      if ( labels_out ) memcpy(labels_out + q * S, labels, (size_t)S);
+   }
+   free(X); free(labels); free(wlabels); free(work);
+   free(wx); free(wy); free(wr);
+   return KINC_OK;
+}
+
 /* ------------------------------------------------- exported test helpers */
 
 float oracle_expf(float x) { return kinc_expf(x); }

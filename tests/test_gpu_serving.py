@@ -1,0 +1,123 @@
+"""Explicit pair lists, batching of work blocks, and the query server.
+
+Results must be bit-exact with the oracle run on the same pair list with the
+same work-block segmentation, and a query's answer must be identical whether
+it runs alone or batched with other queries.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from kinc_amd import _native, gemgen, serving
+from parity_util import assert_block_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _segments(n, rng, blocks=4):
+    cuts = np.sort(rng.choice(np.arange(1, n), size=blocks - 1, replace=False))
+    seg = np.zeros(n, np.int32)
+    for c in cuts:
+        seg[c:] = c
+    return seg
+
+
+@pytest.mark.parametrize("flags", [
+    dict(clusmethod="none", corrmethod="pearson"),
+    dict(clusmethod="gmm", corrmethod="pearson", crit="BIC"),
+    dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True),
+], ids=["pearson", "gmm_pearson", "gmm_spearman"])
+def test_pair_list_with_segments(flags):
+    rng = np.random.default_rng(5)
+    x = gemgen.modules(60, 220, seed=21, nan_rate=0.12)
+    G = x.shape[0]
+    n = 600 if flags["clusmethod"] == "none" else 160
+    a = rng.integers(1, G, n)
+    b = (rng.random(n) * a).astype(np.int64)
+    pairs = np.stack([a, b], 1).astype(np.int32)
+    seg = _segments(n, rng)
+    p = _native.make_params(batch_capacity=65536, **flags)
+    want = oracle.execute_pairs(p, x, pairs, seg)
+    eng = _native.Engine(x, p)
+    try:
+        got = eng.execute_pairs(pairs, seg)
+    finally:
+        eng.close()
+    print("pair list", flags, "pairs", n, "segments", len(np.unique(seg)), flush=True)
+    assert_block_equal(got, want, x.shape[1], where="pair list %s" % flags)
+
+
+def _oracle_hits(p, x, genes, min_abs):
+    pairs = serving.query_pairs(genes, x.shape[0])
+    n = pairs.shape[0]
+    seg = (np.arange(n) // 32768 * 32768).astype(np.int32)
+    K, _, corr, _ = oracle.execute_pairs(p, x, pairs, seg, want_labels=False)
+    rows = []
+    for q in range(n):
+        c = 0
+        for k in range(K[q]):
+            r = corr[q, k]
+            if not np.isnan(r) and min_abs <= abs(r) <= p.max_corr:
+                rows.append((pairs[q, 0], pairs[q, 1], c, r))
+                c += 1
+    return np.array(rows, serving.HIT_DTYPE)
+
+
+def _same_hits(a, b):
+    return (a.shape == b.shape and np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+            and np.array_equal(a["cluster"], b["cluster"])
+            and np.array_equal(a["corr"].view(np.uint32), b["corr"].view(np.uint32)))
+
+
+def test_server_alone_equals_mix_and_oracle():
+    x = gemgen.modules(400, 180, seed=22, nan_rate=0.08)
+    p = _native.make_params(clusmethod="gmm", corrmethod="pearson", crit="BIC")
+    srv = serving.Server(x, p, hbm_budget=64 << 20, batch_pairs=65536)
+    try:
+        q_small = [5]
+        q_mid = [17, 3, 250]
+        q_big = list(range(100, 220, 1))  # 120 genes: about 40k pairs, two work blocks
+        # alone
+        alone = {}
+        for name, genes in (("small", q_small), ("mid", q_mid), ("big", q_big)):
+            qid = srv.submit(genes, 0.5)
+            assert srv.wait(qid, 300)
+            alone[name] = srv.result(qid)
+            print("alone", name, "pairs", alone[name][2], "hits", len(alone[name][0]),
+                  "latency %.3fs" % alone[name][1], flush=True)
+        # mixed: all in flight together, in a different order
+        ids = {name: srv.submit(genes, 0.5)
+               for name, genes in (("big", q_big), ("small", q_small), ("mid", q_mid))}
+        for name, qid in ids.items():
+            assert srv.wait(qid, 300)
+            hits, lat, npairs = srv.result(qid)
+            print("mix", name, "hits", len(hits), "latency %.3fs" % lat, flush=True)
+            assert npairs == alone[name][2]
+            assert _same_hits(hits, alone[name][0]), name
+        pairs_done, batches, busy = srv.stats()
+        assert pairs_done == 2 * sum(v[2] for v in alone.values())
+    finally:
+        srv.close()
+    for name, genes in (("small", q_small), ("mid", q_mid)):
+        want = _oracle_hits(p, x, genes, np.float32(0.5))
+        assert _same_hits(alone[name][0], want), name
+
+
+def test_server_admission_under_small_budget():
+    """A budget that holds one query's results at a time still serves every
+    query (later ones wait for admission)."""
+    x = gemgen.latent(300, 120, seed=23)
+    p = _native.make_params(clusmethod="none", corrmethod="pearson")
+    per_pair = 1 + 4 * p.max_clusters
+    budget = max(1 << 20, 2 * 300 * per_pair)
+    srv = serving.Server(x, p, hbm_budget=budget, batch_pairs=32768)
+    try:
+        ids = [srv.submit([g], 0.3) for g in range(0, 40, 4)]
+        res = []
+        for qid in ids:
+            assert srv.wait(qid, 120)
+            res.append(srv.result(qid))
+        want = _oracle_hits(p, x, [8], np.float32(0.3))
+        assert _same_hits(res[2][0], want)
+    finally:
+        srv.close()

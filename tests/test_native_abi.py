@@ -36,8 +36,8 @@ def test_library_exports(name, native):
 
 
 def test_binding_covers_headers():
-    from kinc_amd import formats
-    bound = {n for n, _, _ in _native.SIGNATURES} | set(formats.SIGNATURES)
+    from kinc_amd import formats, serving
+    bound = {n for n, _, _ in _native.SIGNATURES} | set(formats.SIGNATURES) | set(serving.SIGNATURES)
     missing = set(declared_functions()) - bound
     assert not missing, missing
 
