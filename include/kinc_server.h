@@ -50,7 +50,8 @@ int kinc_server_create(const kinc_sim_params* params, const float* emx, int32_t 
                        int32_t sample_size, int32_t device, int64_t hbm_budget_bytes,
                        int32_t batch_pairs, kinc_server** out);
 
-/* Queue a query; returns its id in *id. */
+/* Queue a query; returns its id in *id.  A query whose result buffer alone
+ * exceeds the HBM budget is refused with KINC_E_NOMEM. */
 int kinc_server_submit(kinc_server* s, const int32_t* genes, int32_t n_genes,
                        float min_abs_corr, int64_t* id);
 

@@ -145,6 +145,7 @@ struct kinc_server
    hipStream_t stream = nullptr;     /* batches */
    hipStream_t copy = nullptr;       /* result copy-back */
    char* d_arena = nullptr;
+   size_t budget = 0;
    std::unique_ptr<Arena> arena;
    Batch batch[2];
 
@@ -155,6 +156,7 @@ struct kinc_server
    std::unordered_map<int64_t, std::shared_ptr<Query>> table;
    int64_t next_id = 1;
    bool stop = false;
+   bool dead = false;                /* the scheduler stopped on a device error */
    std::thread worker;
    std::string err;
 
@@ -401,6 +403,7 @@ static void scheduler(kinc_server* s)
    {
       std::lock_guard<std::mutex> g(s->mu);
       s->err = e;
+      s->dead = true;
       for ( auto& q : s->active ) q->state = -1;
       for ( auto& q : s->pending ) q->state = -1;
       s->cv.notify_all();
@@ -442,6 +445,7 @@ extern "C" int kinc_server_create(const kinc_sim_params* params, const float* em
    CR(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
    CR(hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking));
    CR(hipMalloc(&s->d_arena, (size_t)hbm_budget_bytes));
+   s->budget = (size_t)hbm_budget_bytes & ~(size_t)255;
    s->arena.reset(new Arena((size_t)hbm_budget_bytes));
    for ( auto& b : s->batch )
    {
@@ -476,8 +480,13 @@ extern "C" int kinc_server_submit(kinc_server* s, const int32_t* genes, int32_t 
    const size_t kbytes = ((size_t)q->n + 255) & ~(size_t)255;
    q->corr_off = kbytes;
    q->bytes = kbytes + sizeof(float) * (size_t)q->n * s->MC;
+   if ( q->bytes > s->budget )
+   {
+      return server_fail(s, KINC_E_NOMEM, "query needs " + std::to_string(q->bytes)
+                                         + " result bytes, more than the HBM budget");
+   }
    std::lock_guard<std::mutex> g(s->mu);
-   if ( !s->err.empty() ) return KINC_E_DEVICE;
+   if ( s->dead ) return KINC_E_DEVICE;
    q->id = s->next_id++;
    s->table[q->id] = q;
    if ( q->n == 0 )

@@ -104,20 +104,23 @@ def test_server_alone_equals_mix_and_oracle():
 
 
 def test_server_admission_under_small_budget():
-    """A budget that holds one query's results at a time still serves every
-    query (later ones wait for admission)."""
-    x = gemgen.latent(300, 120, seed=23)
+    """A 1 MiB budget holds one 60-gene query's results (about 34k pairs x
+    21 bytes) at a time: the others wait for admission and are still served,
+    and a query larger than the whole budget is refused."""
+    x = gemgen.latent(600, 120, seed=23)
     p = _native.make_params(clusmethod="none", corrmethod="pearson")
-    per_pair = 1 + 4 * p.max_clusters
-    budget = max(1 << 20, 2 * 300 * per_pair)
-    srv = serving.Server(x, p, hbm_budget=budget, batch_pairs=32768)
+    srv = serving.Server(x, p, hbm_budget=1 << 20, batch_pairs=32768)
     try:
-        ids = [srv.submit([g], 0.3) for g in range(0, 40, 4)]
+        queries = [list(range(i, i + 60)) for i in (0, 200, 400)]
+        ids = [srv.submit(q, 0.3) for q in queries]
         res = []
         for qid in ids:
             assert srv.wait(qid, 120)
             res.append(srv.result(qid))
-        want = _oracle_hits(p, x, [8], np.float32(0.3))
-        assert _same_hits(res[2][0], want)
+        assert all(r[2] == 60 * 599 - 60 * 59 // 2 for r in res)
+        with pytest.raises(_native.KincError):
+            srv.submit(list(range(200)), 0.3)
     finally:
         srv.close()
+    want = _oracle_hits(p, x, queries[1], np.float32(0.3))
+    assert _same_hits(res[1][0], want)
