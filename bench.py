@@ -14,7 +14,11 @@ fraction of the dominant pair kernel, timed with HIP events on the stream it
 runs on, and the CPU oracle timed on a bounded sample of the same pairs on
 this host (which also re-checks bit-exact parity on that sample).
 
-Usage: python bench.py [--gpus 1] [--steps 8] [--warmup 1]
+A serving leg follows: a Poisson stream of gene-set queries against a
+resident 60,000 x 475 matrix (GMM + Pearson, |r| >= 0.5), with p50/p99
+latency beside pairs/s, and the latency of one 100-gene query alone.
+
+Usage: python bench.py [--gpus 1] [--steps 8] [--warmup 1] [--no-serve]
 """
 import argparse
 import json
@@ -155,6 +159,78 @@ def cpu_baseline(oracle, native, x, flags, blocks, seconds, gpu_out):
     return done / t_used, done, t_used, checked, mismatches
 
 
+def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
+    """Poisson query stream against the resident 60,000 x 475 matrix (GMM +
+    Pearson, |r| >= 0.5): query sizes log-uniform on [1, max_genes] genes,
+    all partners of each query gene; latency is submit-to-answer as measured
+    by the server.  Returns the `serving` object of the JSON line."""
+    import threading
+    from kinc_amd import serving
+    cfg = gemgen.CONFIGS["D"]
+    t = time.time()
+    x = gemgen.make("D")
+    G = x.shape[0]
+    log("serving: generated %dx%d GEM in %.1fs" % (G, x.shape[1], time.time() - t))
+    p = native.make_params(**cfg["params"])
+    srv = serving.Server(x, p, hbm_budget=8 << 30, batch_pairs=131072)
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(0.0, np.log(max_genes), n_queries)).astype(int).clip(1, max_genes)
+    queries = [rng.choice(G, size=int(k), replace=False) for k in sizes]
+    gaps = rng.exponential(1.0 / qps, n_queries)
+    ids = []
+    lock = threading.Lock()
+
+    def producer():
+        for qgenes, gap in zip(queries, gaps):
+            time.sleep(gap)
+            qid = srv.submit(qgenes, 0.5)
+            with lock:
+                ids.append(qid)
+
+    p0, b0, busy0 = srv.stats()
+    t0 = time.perf_counter()
+    th = threading.Thread(target=producer)
+    th.start()
+    lat, npairs, nhits = [], 0, 0
+    done = 0
+    while done < n_queries:
+        with lock:
+            pending = ids[done:]
+        if not pending:
+            time.sleep(0.01)
+            continue
+        qid = pending[0]
+        assert srv.wait(qid, 1200)
+        hits, l, n = srv.result(qid)
+        lat.append(l)
+        npairs += n
+        nhits += len(hits)
+        done += 1
+        log("  query %d/%d: %d genes, %d pairs, %d hits, %.3f s" % (done, n_queries, sizes[done - 1],
+                                                                   n, len(hits), l))
+    th.join()
+    wall = time.perf_counter() - t0
+    p1, b1, busy1 = srv.stats()
+    out = {
+        "workload": "60000 genes x 475 samples FP32 (module generator, 8% NaN), GMM K=1..5 BIC + "
+                    "Pearson, |r| >= 0.5, all partners of each query gene",
+        "queries": n_queries, "qps": qps, "query_genes": "log-uniform on [1, %d]" % max_genes,
+        "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+        "max_s": float(np.max(lat)),
+        "pairs_per_s": (p1 - p0) / wall, "device_busy_frac": (busy1 - busy0) / wall,
+        "pairs": int(npairs), "hits": int(nhits), "batches": int(b1 - b0),
+    }
+    if big_query:
+        qgenes = rng.choice(G, size=big_query, replace=False)
+        qid = srv.submit(qgenes, 0.5)
+        assert srv.wait(qid, 1200)
+        hits, l, n = srv.result(qid)
+        log("  %d-gene query alone: %d pairs, %.2f s" % (big_query, n, l))
+        out["single_query"] = {"genes": big_query, "pairs": int(n), "latency_s": l}
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +239,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-queries", type=int, default=24)
+    ap.add_argument("--serve-qps", type=float, default=0.5)
+    ap.add_argument("--serve-max-genes", type=int, default=16)
+    ap.add_argument("--serve-big-query", type=int, default=100)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -245,6 +326,11 @@ def main():
                          "C oracle (-O2, no FP contraction); GPU output bit-exact on %d/%d"
                          % (n, len(blocks), checked - bad, checked)}
 
+    serving_obj = None
+    if not args.no_serve:
+        serving_obj = serve_leg(native, gemgen, args.serve_queries, args.serve_qps,
+                                args.serve_max_genes, args.serve_big_query)
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -281,6 +367,7 @@ def main():
             "kernel_time_share": kernel_ms_total / (secs * 1e3),
         },
         "cpu_baseline": cpu,
+        "serving": serving_obj,
     }
     print(json.dumps(line), flush=True)
 
