@@ -310,7 +310,6 @@ def main():
     dom = max(prof.items(), key=lambda kv: kv[1][1])[0]
     launches, dom_ms = prof[dom]
     achieved = flops_by_P.get(dom, 0.0) / launches / (dom_ms / launches * 1e-3) / 1e12
-    kernel_ms_total = sum(v[1] for v in prof.values())
     whole_tflops = flops_total / secs / 1e12
 
     cpu = None
@@ -364,7 +363,6 @@ def main():
             "traffic": TRAFFIC_BYTES_PER_LAUNCH,
             "traffic_source": TRAFFIC_SOURCE,
             "whole_job_tflops": whole_tflops,
-            "kernel_time_share": kernel_ms_total / (secs * 1e3),
         },
         "cpu_baseline": cpu,
         "serving": serving_obj,

@@ -31,6 +31,7 @@
 #include <math.h>
 
 #if defined(__HIPCC__) || defined(__HIP_DEVICE_COMPILE__)
+#include <hip/hip_runtime.h>
 #define KINC_HD __host__ __device__ __forceinline__
 #else
 #define KINC_HD static inline
@@ -99,6 +100,17 @@ KINC_HD float kinc_frexpf(float x, int* e)
 #endif
 }
 
+/* min(max(x, lo), hi) for x that is not NaN (a NaN x gives an unspecified
+ * value: callers replace the result for NaN input). */
+KINC_HD float kinc_clampf(float x, float lo, float hi)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   return __builtin_amdgcn_fmed3f(x, lo, hi);
+#else
+   return fminf(fmaxf(x, lo), hi);
+#endif
+}
+
 KINC_HD float kinc_expf(float x)
 {
    /* Outside [-104, 89] the result is 0 or +inf; clamping there lets the
@@ -107,7 +119,7 @@ KINC_HD float kinc_expf(float x)
    const float LOG2EF = 1.44269504088896341f;
    const float C1 = 0.693359375f;
    const float C2 = -2.12194440e-4f;
-   float xc = fminf(fmaxf(x, -104.0f), 89.0f);
+   float xc = kinc_clampf(x, -104.0f, 89.0f);
 
    float nf = kinc_rintf(xc * LOG2EF);
    float g = kinc_fmaf(-nf, C1, xc);
@@ -132,12 +144,10 @@ KINC_HD float kinc_logf(float x)
    const float C1 = 0.693359375f;
    const float C2 = -2.12194440e-4f;
 
-   /* m in [0.5, 1); zero, infinities and NaN are replaced at the end */
+   /* m in [0.5, 1) for positive finite x; the values computed for zero,
+    * negative, infinite and NaN x are replaced at the end */
    int e = 0;
    float m = kinc_frexpf(x, &e);
-   int fin = (x > 0.0f) && (x < INFINITY);
-   m = fin ? m : 0.75f;
-   e = fin ? e : 0;
 
    int lo = m < SQRTHF;
    e = lo ? e - 1 : e;
@@ -162,11 +172,10 @@ KINC_HD float kinc_logf(float x)
    float r = f + y;
    r = kinc_fmaf(fe, C1, r);
 
-   r = (x == INFINITY) ? INFINITY : r;
-   r = (x == 0.0f) ? -INFINITY : r;
-   r = (x < 0.0f) ? NAN : r;
-   r = (x != x) ? x : r;
-   return r;
+   /* log(+inf) = +inf, log(+-0) = -inf, log(x < 0) = log(NaN) = NaN */
+   const int regular = (x > 0.0f) && (x < INFINITY);
+   const float special = (x == 0.0f) ? -INFINITY : ((x > 0.0f) ? x : NAN);
+   return regular ? r : special;
 }
 
 #endif
