@@ -37,9 +37,10 @@ SOURCES = [
     "server.cpp",
 ]
 # pair-kernel instances (KMAX, P): register-slot counts P, and P = 0 for the
-# memory-store instance used above 1024 samples
+# memory-store instance used above the register range (1024 samples for up to
+# 5 clusters, 512 for up to 8)
 INSTANCES = ([(5, p) for p in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16)]
-             + [(8, p) for p in (0, 2, 4, 8, 16)])
+             + [(8, p) for p in (0, 2, 4, 8)])
 CLI_SOURCE = "kinc_cli.cpp"
 CLI = os.path.join(HERE, "kinc")
 
@@ -55,9 +56,10 @@ def _compile(src, defines=(), tag=None):
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    jobs = [(src, (), None) for src in SOURCES]
-    jobs += [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p),
-              "pairs_k%d_p%d" % (k, p)) for k, p in INSTANCES]
+    inst = sorted(INSTANCES, key=lambda kp: -(kp[0] * (kp[1] or 4)))  # longest first
+    jobs = [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p),
+             "pairs_k%d_p%d" % (k, p)) for k, p in inst]
+    jobs += [(src, (), None) for src in SOURCES]
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))

@@ -25,7 +25,7 @@
 
 #define KINC_FOR_EACH_INSTANCE(X)                                                   \
    X(5, 0) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(5, 10)   \
-   X(5, 12) X(5, 14) X(5, 16) X(8, 0) X(8, 2) X(8, 4) X(8, 8) X(8, 16)
+   X(5, 12) X(5, 14) X(5, 16) X(8, 0) X(8, 2) X(8, 4) X(8, 8)
 
 extern "C" {
 #define KINC_DECL(K, P) const void* kinc_pair_kernel_k##K##_p##P();
@@ -42,23 +42,24 @@ namespace {
 /* Register-slot counts that have a kernel instance (64*P samples per pair),
  * per cluster bound KMAX. */
 const int kSlotList5[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16};
-const int kSlotList8[] = {2, 4, 8, 16};
+const int kSlotList8[] = {2, 4, 8};
 const int kMaxBins = 12;
 
 thread_local std::string g_create_error;
 
 /* Slot classes used for a handle: every instance up to the one S needs,
- * or the memory-store instance (P = 0) alone above the register range. */
+ * or the memory-store instance (P = 0) alone above the register range
+ * (1,024 samples for up to 5 clusters, 512 for up to 8). */
 std::vector<int> slot_classes(int S, int kmax)
 {
    std::vector<int> out;
-   if ( S > KINC_MAX_REG_SAMPLES )
+   const int* list = kmax == 5 ? kSlotList5 : kSlotList8;
+   const int n = kmax == 5 ? 12 : 3;
+   if ( S > 64 * list[n - 1] )
    {
       out.push_back(0);
       return out;
    }
-   const int* list = kmax == 5 ? kSlotList5 : kSlotList8;
-   const int n = kmax == 5 ? 12 : 4;
    for ( int i = 0; i < n; ++i )
    {
       out.push_back(list[i]);
@@ -327,7 +328,7 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    off = align16(off + (size_t)S);
    a.lds_w = (int32_t)off;
    off = align16(off + (size_t)S);
-   if ( S <= KINC_MAX_REG_SAMPLES )
+   if ( h->classes[0] != 0 )
    {
       /* everything in LDS; the chain tile shares the sort-buffer region */
       a.scratch = nullptr;

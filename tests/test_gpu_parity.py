@@ -31,6 +31,8 @@ CASES = [
      dict(clusmethod="kmeans", corrmethod="spearman", crit="BIC", minexpr=7.0), 0, 150),
     ("gmm_maxclus8_small_S", dict(kind="modules", G=40, S=64, seed=18, nan_rate=0.05),
      dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", maxclus=8, minclus=2), 0, 200),
+    ("gmm_maxclus8_S700_memstore", dict(kind="modules", G=16, S=700, seed=24, nan_rate=0.10),
+     dict(clusmethod="gmm", corrmethod="pearson", crit="BIC", maxclus=7), 0, 120),
     # above 1024 samples: the memory-store instance
     ("gmm_spearman_S1500", dict(kind="modules", G=14, S=1500, seed=19, nan_rate=0.10),
      dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True), 0, 91),
