@@ -102,9 +102,26 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
    return v;
 }
 
-__device__ __forceinline__ void wave_sync()
+/*
+ * Make this wave's earlier writes to its working set visible to its later
+ * reads by other lanes.  Each workgroup is one wave.  LDS operations of a
+ * wave complete in execution order, so for an LDS working set ordering the
+ * compiler is enough (a wavefront-scope fence); a working set in global
+ * memory (memory-store instance) takes the full workgroup barrier with its
+ * memory-counter waits.
+ */
+__device__ __forceinline__ void wave_sync(bool global_mem)
 {
-   __syncthreads();
+   if ( global_mem )
+   {
+      __syncthreads();
+   }
+   else
+   {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+   }
 }
 
 __device__ __forceinline__ int next_pow2(int n)
@@ -213,7 +230,7 @@ __device__ int fetch_pair(const SimArgs& a, int gx, int gy, const Lds& L)
       }
       base += popc(m);
    }
-   wave_sync();
+   wave_sync(L.lgbuf != nullptr);
    return base;
 }
 
@@ -227,7 +244,7 @@ __device__ int fetch_pair(const SimArgs& a, int gx, int gy, const Lds& L)
  * across lanes gives the same result as the serial loop.
  */
 template <bool PAYLOAD>
-__device__ void bitonic_sort(float* key, float* pay, int size)
+__device__ void bitonic_sort(float* key, float* pay, int size, bool global_mem)
 {
    const int lane = lane_id();
    const int half = size >> 1;
@@ -257,7 +274,7 @@ __device__ void bitonic_sort(float* key, float* pay, int size)
                }
             }
          }
-         wave_sync();
+         wave_sync(global_mem);
       }
    }
 }
@@ -288,19 +305,19 @@ __device__ void mark_outliers(const Lds& L, int N, int j, int8_t cluster, int8_t
    }
    if ( n == 0 )
    {
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
       return;
    }
    int n2 = next_pow2(n);
    for ( int i = n + lane; i < n2; i += WAVE ) key[i] = INFINITY;
-   wave_sync();
-   bitonic_sort<false>(key, nullptr, n2);
+   wave_sync(L.lgbuf != nullptr);
+   bitonic_sort<false>(key, nullptr, n2, L.lgbuf != nullptr);
 
    const float Q1 = key[n * 1 / 4];
    const float Q3 = key[n * 3 / 4];
    const float T_min = Q1 - 1.5f * (Q3 - Q1);
    const float T_max = Q3 + 1.5f * (Q3 - Q1);
-   wave_sync();
+   wave_sync(L.lgbuf != nullptr);
 
    for ( int i = lane; i < N; i += WAVE )
    {
@@ -311,7 +328,7 @@ __device__ void mark_outliers(const Lds& L, int N, int j, int8_t cluster, int8_t
          L.lab[i] = marker;
       }
    }
-   wave_sync();
+   wave_sync(L.lgbuf != nullptr);
 }
 
 /* pairwise_clustering.cpp:77-84: labels[i] >= 0 (i < N) take the fit's
@@ -329,7 +346,7 @@ __device__ void copy_back(const Lds& L, int N)
       if ( ge ) L.lab[i] = L.w[jb + popc(m & lanes_below())];
       jb += popc(m);
    }
-   wave_sync();
+   wave_sync(L.lgbuf != nullptr);
 }
 
 /* --------------------------------------------------------- chain helpers */
@@ -577,7 +594,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          *(float4*)p = make_float4(c.x, c.y, a00, a01);
          *(float4*)(p + 4) = make_float4(a10, a11, nrm, kinc_logf(1.0f / K));
       }
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
    }
 
    float prevLogL = -INFINITY;
@@ -629,11 +646,11 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
                tile[(1 + K + 2 * k) * STR + lane] = gm * xa;
                tile[(2 + K + 2 * k) * STR + lane] = gm * xb;
             }
-            wave_sync();
+            wave_sync(L.lgbuf != nullptr);
             int len = N - m * WAVE;
             len = len < TILE ? len : TILE;
             acc = chain_add(tile, STR, 1 + 3 * K, len, acc);
-            wave_sync();
+            wave_sync(L.lgbuf != nullptr);
          }
       }
 
@@ -669,7 +686,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       const float cm = 1.0f / Gam;
       const float mu0 = mxs * cm;
       const float mu1 = mys * cm;
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
       if ( lane < K )
       {
          float* p = prm + 8 * lane;
@@ -677,7 +694,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          p[PRM_MU1] = mu1;
          p[PRM_LPI] = lpk + (logGamma - logGammaSum);
       }
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
 
       /* pass B: chain 3k / 3k+1 / 3k+2 = Sigma_k entries 00, 01 (= 10), 11 */
       float accB = 0.0f;
@@ -706,11 +723,11 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
                tile[(3 * k + 1) * STR + lane] = gm * (xm0 * xm1);
                tile[(3 * k + 2) * STR + lane] = gm * (xm1 * xm1);
             }
-            wave_sync();
+            wave_sync(L.lgbuf != nullptr);
             int len = N - m * WAVE;
             len = len < TILE ? len : TILE;
             accB = chain_add(tile, STR, 3 * K, len, accB);
-            wave_sync();
+            wave_sync(L.lgbuf != nullptr);
          }
       }
 
@@ -729,13 +746,13 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          p[PRM_A11] = a11;
          p[PRM_NRM] = nrm;
       }
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
    }
 
    out.iters = (uint16_t)((e_steps & 0x0fff) | (converged ? (1u << 12) : 0u) | (failed ? (1u << 13) : 0u));
    if ( failed )
    {
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
       out.ok = false;
       return out;
    }
@@ -773,7 +790,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          if ( want_entropy ) st.LPX(m) = (max_k >= 0) ? kinc_logf(mg) : NAN;
       }
    }
-   wave_sync();
+   wave_sync(L.lgbuf != nullptr);
 
    if ( want_entropy )
    {
@@ -784,11 +801,11 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          if ( m < nslots )
          {
             tile[lane] = st.LPX(m);
-            wave_sync();
+            wave_sync(L.lgbuf != nullptr);
             int len = N - m * WAVE;
             len = len < TILE ? len : TILE;
             accE = chain_add(tile, STR, 1, len, accE);
-            wave_sync();
+            wave_sync(L.lgbuf != nullptr);
          }
       }
       out.E = bcast(accE, 0);
@@ -833,9 +850,9 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
          int i = lcg(rs) % N;
          my = (lane == k) ? i : my;
       }
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
       if ( lane < K ) *(float2*)(means + 2 * lane) = L.X[my];
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
 #pragma unroll
       for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
       {
@@ -901,11 +918,11 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
                   tile[(2 * k) * STR + lane] = mem ? xa : 0.0f;
                   tile[(2 * k + 1) * STR + lane] = mem ? xb : 0.0f;
                }
-               wave_sync();
+               wave_sync(L.lgbuf != nullptr);
                int len = N - m * WAVE;
                len = len < TILE ? len : TILE;
                acc = chain_add(tile, STR, 2 * K, len, acc);
-               wave_sync();
+               wave_sync(L.lgbuf != nullptr);
             }
          }
          int nk = 0;
@@ -924,7 +941,7 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
          const float sy = __shfl(acc, 2 * lane + 1);
          const float c = 1.0f / (float)nk;
          if ( lane < K ) *(float2*)(means + 2 * lane) = make_float2(sx * c, sy * c);
-         wave_sync();
+         wave_sync(L.lgbuf != nullptr);
       }
 
       /* computeLogLikelihood (pairwise_kmeans.cpp:106-127): one chain over
@@ -953,14 +970,14 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
          }
          kbase += cnt;
       }
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
       float Ssum = 0.0f;
       if ( lane == 0 )
       {
          for ( int j = 0; j < kbase; ++j ) Ssum = Ssum + seq[j];
       }
       const float logL = -bcast(Ssum, 0);
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
 
       if ( bestLogL < logL )
       {
@@ -971,7 +988,7 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
             const int i = lane + m * WAVE;
             if ( m < nslots && i < N ) L.w[i] = st.Y(m);
          }
-         wave_sync();
+         wave_sync(L.lgbuf != nullptr);
       }
    }
    out.logL = bestLogL;
@@ -1081,7 +1098,7 @@ __device__ void fill_stale(const SimArgs& a, const Lds& L, int q, int N)
       v = hi;
       qp = d;
    }
-   wave_sync();
+   wave_sync(L.lgbuf != nullptr);
 }
 
 /* -------------------------------------------------------------- Pearson */
@@ -1170,7 +1187,7 @@ __device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
    }
    if ( n < a.min_samples )
    {
-      wave_sync();
+      wave_sync(L.lgbuf != nullptr);
       return NAN;
    }
    const int n2 = next_pow2(n);
@@ -1180,9 +1197,9 @@ __device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
       ky[i] = INFINITY;
       kr[i] = 0.0f;
    }
-   wave_sync();
-   bitonic_sort<true>(kx, ky, n2);
-   bitonic_sort<true>(ky, kr, n2);
+   wave_sync(L.lgbuf != nullptr);
+   bitonic_sort<true>(kx, ky, n2, L.lgbuf != nullptr);
+   bitonic_sort<true>(ky, kr, n2, L.lgbuf != nullptr);
 
    uint32_t d = 0;
    for ( int i = lane; i < n; i += WAVE )
@@ -1191,7 +1208,7 @@ __device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
       d += (uint32_t)(tmp * tmp);
    }
    d = wave_sum_u32(d);
-   wave_sync();
+   wave_sync(L.lgbuf != nullptr);
    const uint32_t un = (uint32_t)n;
    const int32_t denom = (int32_t)(un * (un * un - 1u));
    return (float)(1.0 - 6.0 * (double)(int32_t)d / (double)denom);
@@ -1340,7 +1357,7 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
       int8_t* lo = a.labels_out + (size_t)q * S;
       for ( int s = lane; s < S; s += WAVE ) lo[s] = L.lab[s];
    }
-   wave_sync();
+   wave_sync(L.lgbuf != nullptr);
 }
 
 } // namespace
