@@ -33,6 +33,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "gene pairs/sec GMM+Spearman 20k×500, batched vs plain; p50/p99 query latency"
 FP32_PEAK_TFLOPS = 157.3
+WORKLOADS = {
+    "H": "KINC similarity, GMM (K=1..5, BIC) + Spearman, pre/post outliers, minsamp 30; "
+         "20000 genes x 500 samples FP32, module generator, 10% NaN",
+    "B": "KINC similarity, GMM (K=1..5, BIC) + Spearman, pre/post outliers, minsamp 30; "
+         "5000 genes x 500 samples FP32, module generator, 10% NaN",
+    "A": "KINC similarity, no clustering, Pearson, minsamp 30; 1000 genes x 200 samples FP32, "
+         "3-factor generator, 5% NaN",
+    "C": "KINC similarity, K-means (K=1..5, BIC) + Pearson, minsamp 30; 20000 genes x 1000 "
+         "samples FP32, module generator, per-gene NaN rate Beta(1,4)",
+}
 # HBM bytes per launch of the dominant pair kernel, from the FETCH_SIZE and
 # WRITE_SIZE passes (scaled by a copy kernel of known size) recorded in:
 TRAFFIC_SOURCE = "profiles/k_pairs_gmm_spearman_20000x500_counters.md"
@@ -44,7 +54,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pair_flops(N, iters, corr_method, labels, K):
+def pair_flops(N, iters, corr_method, labels, K, clus="gmm"):
     """Algorithmic FP32 flops of one pair from its per-fit iteration records.
 
     GMM fit with K components on N points, T E-steps and T_M M-steps:
@@ -58,9 +68,14 @@ def pair_flops(N, iters, corr_method, labels, K):
         if it == 0:
             continue
         T = it & 0xFFF
+        Kc = k + 1
+        if clus == "kmeans":
+            # per Lloyd iteration: K distances (2 sub, 2 mul, 2 add) per point
+            # and the member sums (2 adds per point)
+            total += N * T * (6 * Kc + 2)
+            continue
         conv = (it >> 12) & 1
         TM = T - 1 if conv else T
-        Kc = k + 1
         total += N * Kc * (18 * T + 20 * TM) + 2 * N * T
     if corr_method == "pearson":
         for k in range(int(K)):
@@ -68,11 +83,11 @@ def pair_flops(N, iters, corr_method, labels, K):
     return total
 
 
-def block_flops(K, labels, iters, corr_method):
+def block_flops(K, labels, iters, corr_method, clus="gmm"):
     N = labels.shape[1] - np.count_nonzero((labels == -9) | (labels == -6), axis=1)
     f = np.zeros(K.shape[0], np.float64)
     for q in range(K.shape[0]):
-        f[q] = pair_flops(int(N[q]), iters[q], corr_method, labels[q], K[q])
+        f[q] = pair_flops(int(N[q]), iters[q], corr_method, labels[q], K[q], clus)
     return f, N
 
 
@@ -130,35 +145,70 @@ def run_form(native, x, flags, form, blocks, warmup, torch):
 
 
 def cpu_baseline(oracle, native, x, flags, blocks, seconds, gpu_out):
-    """Oracle (single thread) on the first pairs of the timed blocks, up to
-    `seconds` of CPU time; returns pairs/s and the bit-exact check."""
+    """Oracle (single thread) on a prefix of every timed block, sized so the
+    whole sample takes about `seconds` of CPU time; each prefix starts at its
+    block's first pair, so block-dependent semantics (Pearson's shared X
+    workspace) match the device run.  Returns pairs/s and the bit-exact check
+    of the sampled pairs against the device outputs."""
     from parity_util import count_mismatching_pairs as compare_block
     p = native.make_params(**flags)
-    G, S = x.shape
     done = 0
     t_used = 0.0
-    per_block = 24
     mismatches = 0
-    checked = 0
-    rounds = 0
-    while t_used < seconds and rounds < 64:
-        for i, b in enumerate(blocks):
-            lo = rounds * per_block
-            if lo >= BLOCK:
-                break
-            start = b * BLOCK + lo
-            t0 = time.process_time()
-            want = oracle.execute_block(p, x, start, per_block)
-            t_used += time.process_time() - t0
-            done += per_block
-            got = tuple(a[i][lo:lo + per_block] for a in
-                        (gpu_out["K"], gpu_out["labels"], gpu_out["corr"], gpu_out["iters"]))
-            mismatches += compare_block(got, want)
-            checked += per_block
-            if t_used >= seconds:
-                break
-        rounds += 1
-    return done / t_used, done, t_used, checked, mismatches
+    per_block = 16
+    for i, b in enumerate(blocks):
+        if i == 1:
+            # size the remaining prefixes from the rate seen on the first one
+            rate = done / max(t_used, 1e-6)
+            per_block = int(max(8, min(BLOCK, rate * (seconds - t_used) / max(1, len(blocks) - 1))))
+        start = b * BLOCK
+        t0 = time.process_time()
+        want = oracle.execute_block(p, x, start, per_block)
+        t_used += time.process_time() - t0
+        done += per_block
+        got = tuple(a[i][:per_block] for a in
+                    (gpu_out["K"], gpu_out["labels"], gpu_out["corr"], gpu_out["iters"]))
+        mismatches += compare_block(got, want)
+    return done / t_used, done, t_used, done, mismatches
+
+
+def host_threads():
+    """Host threads the all-core CPU leg may use: the affinity set, at most 16
+    (a one-GPU share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline_threads(oracle, native, x, flags, blocks, seconds, gpu_out, rate1, threads):
+    """The oracle on every host thread at once, one work-block prefix per
+    thread (the reference parallelises by blocks too), sized from the
+    single-thread rate so the leg takes about `seconds` of wall time.  The
+    prefixes are those of the timed blocks, so the device outputs check them
+    bit for bit.  Returns pairs/s (wall), pairs, wall seconds, mismatches."""
+    from concurrent.futures import ThreadPoolExecutor
+    from parity_util import count_mismatching_pairs as compare_block
+    p = native.make_params(**flags)
+    tasks = [(i, b) for i, b in enumerate(blocks)][:threads]
+    per_block = int(max(8, min(BLOCK, rate1 * seconds)))
+
+    def work(ib):
+        i, b = ib
+        return i, oracle.execute_block(p, x, b * BLOCK, per_block)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=len(tasks)) as ex:
+        results = list(ex.map(work, tasks))
+    wall = time.perf_counter() - t0
+    mismatches = 0
+    for i, want in results:
+        got = tuple(a[i][:per_block] for a in
+                    (gpu_out["K"], gpu_out["labels"], gpu_out["corr"], gpu_out["iters"]))
+        mismatches += compare_block(got, want)
+    done = per_block * len(tasks)
+    return done / wall, done, wall, mismatches, len(tasks)
 
 
 def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
@@ -241,6 +291,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-threads", action="store_true",
+                    help="time the CPU oracle on one thread only")
+    ap.add_argument("--config", default="H", choices=["A", "B", "H", "C"],
+                    help="workload (H is the headline; the others are reported for comparison)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=24)
     ap.add_argument("--serve-qps", type=float, default=0.5)
@@ -276,15 +330,17 @@ def main():
     torch.cuda.set_device(0)
     native.lib()
 
-    cfg = gemgen.CONFIGS["H"]
+    cfg = gemgen.CONFIGS[args.config]
     flags = dict(cfg["params"])
     t = time.time()
-    x = gemgen.make("H")
+    x = gemgen.make(args.config)
     G, S = x.shape
     log("generated %dx%d GEM in %.1fs" % (G, S, time.time() - t))
     total_pairs = native.pair_count(G)
     nblocks = (total_pairs + BLOCK - 1) // BLOCK
-    blocks = [(97 * (i + 1)) % nblocks for i in range(args.steps)]
+    blocks = sorted({(97 * (i + 1)) % nblocks for i in range(args.steps)})
+    if len(blocks) < args.steps:
+        blocks = list(range(min(args.steps, nblocks)))
 
     pairs, secs, prof, out = run_form(native, x, flags, "persistent", blocks, args.warmup, torch)
     value = pairs / secs
@@ -304,9 +360,10 @@ def main():
     flops_total = 0.0
     flops_by_P = {}
     for i in range(len(blocks)):
-        f, N = block_flops(out["K"][i], out["labels"][i], out["iters"][i], flags["corrmethod"])
+        f, N = block_flops(out["K"][i], out["labels"][i], out["iters"][i], flags["corrmethod"],
+                           flags.get("clusmethod", "none"))
         flops_total += f.sum()
-        Ps = np.array([slots_for(int(n)) for n in N])
+        Ps = np.array([slots_for(int(n), S) for n in N])
         for P in np.unique(Ps):
             flops_by_P[int(P)] = flops_by_P.get(int(P), 0.0) + float(f[Ps == P].sum())
     # the dominant kernel is the bin instance that carries the most work; bins
@@ -325,9 +382,21 @@ def main():
         log("cpu oracle: %d pairs in %.1fs cpu -> %.2f pairs/s; parity %d/%d mismatching"
             % (n, used, rate, bad, checked))
         cpu = {"value": rate, "unit": "pairs/s", "cores": 1, "kind": "port",
-               "sample": "%d pairs: the first pairs of the %d timed work blocks, single-threaded "
+               "sample": "%d pairs: a prefix of each of the %d timed work blocks, single-threaded "
                          "C oracle (-O2, no FP contraction); GPU output bit-exact on %d/%d"
                          % (n, len(blocks), checked - bad, checked)}
+        if not args.no_cpu_threads:
+            threads = min(host_threads(), len(blocks))
+            if threads > 1:
+                mrate, mn, mwall, mbad, used_t = cpu_baseline_threads(
+                    oracle, native, x, flags, blocks, args.cpu_seconds * 2 / 3, out, rate, threads)
+                log("cpu oracle, %d threads: %d pairs in %.1fs wall -> %.2f pairs/s; parity %d/%d "
+                    "mismatching" % (used_t, mn, mwall, mrate, mbad, mn))
+                cpu = {"value": mrate, "unit": "pairs/s", "cores": used_t, "kind": "port",
+                       "sample": "%d pairs: one %d-pair prefix of a timed work block per thread, "
+                                 "C oracle (-O2, no FP contraction) on %d host threads; GPU output "
+                                 "bit-exact on %d/%d" % (mn, mn // used_t, used_t, mn - mbad, mn),
+                       "single_thread": {"value": rate, "cores": 1, "sample": cpu["sample"]}}
 
     serving_obj = None
     if not args.no_serve:
@@ -348,8 +417,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": "KINC similarity, GMM (K=1..5, BIC) + Spearman, pre/post outliers, "
-                        "minsamp 30; 20000 genes x 500 samples FP32, module generator, 10% NaN",
+            "workload": WORKLOADS[args.config],
             "pairs_per_step": BLOCK,
             "step": "one 32768-pair work block",
             "form": "persistent",
@@ -360,6 +428,8 @@ def main():
         "roofline": {
             "bound": "valu",
             "kernel": "k_pairs<P=%d,KMAX=5> (persistent form)" % dom,
+            "flops_formula": "GMM: N*K*(18T + 20T_M) + 2NT per fit; K-means: N*T*(6K + 2); "
+                             "Pearson: 8 per in-cluster point",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",

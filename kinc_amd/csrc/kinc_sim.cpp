@@ -318,7 +318,7 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    a.tile_stride = 64 + 4;
    a.mem_slots = (S + 63) / 64;
    const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
-   const size_t sort_bytes = sizeof(float) * 3 * (size_t)a.np2;
+   const size_t sort_bytes = sizeof(float) * 2 * (size_t)a.np2;
 
    /* per-pair arrays: X, labels, work labels, then sort buffers */
    size_t off = 0;

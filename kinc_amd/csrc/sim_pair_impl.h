@@ -40,9 +40,9 @@
 #define TILE 64
 
 /* One wave per workgroup; the second bound asks for at least KINC_PAIR_WAVES
- * waves per SIMD, which caps the VGPR budget of the pair kernels. */
+ * waves per SIMD, which caps the VGPR budget of the pair kernels (512 / W). */
 #ifndef KINC_PAIR_WAVES
-#define KINC_PAIR_WAVES 2
+#define KINC_PAIR_WAVES 4
 #endif
 #define KINC_PAIR_BOUNDS __launch_bounds__(64, KINC_PAIR_WAVES)
 
@@ -1160,9 +1160,11 @@ __device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
 {
    const int S = a.S;
    const int lane = lane_id();
+   /* x, y in two buffers; the rank payload of the second sort reuses the
+    * x buffer, which the first sort leaves free */
    float* kx = L.u;
    float* ky = L.u + a.np2;
-   float* kr = L.u + 2 * a.np2;
+   float* kr = kx;
    int n = 0;
    int jb = 0;
    for ( int i0 = 0; i0 < S; i0 += WAVE )
@@ -1180,7 +1182,6 @@ __device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
          float2 v = L.X[j];
          kx[pos] = v.x;
          ky[pos] = v.y;
-         kr[pos] = (float)(pos + 1);
       }
       jb += popc(mge);
       n += popc(min);
@@ -1195,10 +1196,13 @@ __device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
    {
       kx[i] = INFINITY;
       ky[i] = INFINITY;
-      kr[i] = 0.0f;
    }
    wave_sync(L.lgbuf != nullptr);
    bitonic_sort<true>(kx, ky, n2, L.lgbuf != nullptr);
+   /* ranks n+1 for the extracted samples in sample order, 0 for the padding
+    * (pairwise_spearman.cpp:45,57) */
+   for ( int i = lane; i < n2; i += WAVE ) kr[i] = (i < n) ? (float)(i + 1) : 0.0f;
+   wave_sync(L.lgbuf != nullptr);
    bitonic_sort<true>(ky, kr, n2, L.lgbuf != nullptr);
 
    uint32_t d = 0;
