@@ -56,6 +56,39 @@ int kinc_emx_read(const char* path, float* data);
  * metadata size (or < 0 on error). */
 int64_t kinc_file_metadata(const char* path, char* buf, int64_t cap);
 
+/* Element-wise transforms of ExpressionMatrix::Transform
+ * (src/core/expressionmatrix.cpp:8-14: "none", "natural logarithm",
+ * "logarithm base 2", "logarithm base 10"). */
+#define KINC_TRANSFORM_NONE 0
+#define KINC_TRANSFORM_NLOG 1
+#define KINC_TRANSFORM_LOG2 2
+#define KINC_TRANSFORM_LOG10 3
+
+/*
+ * ImportExpressionMatrix::process (src/core/importexpressionmatrix.cpp:20-170):
+ * whitespace-delimited text to EMX.  With sample_size == 0 the first line
+ * holds the sample names, otherwise samples are named 0..sample_size-1 and
+ * every line is data.  A data line is a gene name and sample_size values; a
+ * value equal to nan_token (NULL or "" matches nothing) is NaN, any other is
+ * read as a double, transformed (log, log2, log10 in double) and stored as
+ * float.  A line with another number of fields, or a value that does not
+ * parse, fails with the reference's message.  threads <= 0 uses every host
+ * thread (at most 16).  gene_size / sample_size_out may be NULL.
+ */
+int kinc_emx_import_text(const char* text_path, const char* emx_path, const char* nan_token,
+                         int32_t sample_size, int32_t transform, int32_t threads,
+                         int32_t* gene_size, int32_t* sample_size_out);
+
+/*
+ * ExportExpressionMatrix::process (src/core/exportexpressionmatrix.cpp:20-99):
+ * EMX to text.  First line: every sample name followed by a tab; then per
+ * gene its name and, per sample, a tab and either nan_token (for NaN) or the
+ * value with the stored transform undone, printed with 12 significant
+ * digits (QTextStream's SmartNotation at precision 12, i.e. "%.12g").
+ * Names come from the file's "genes"/"samples" metadata, else 0, 1, 2, ...
+ */
+int kinc_emx_export_text(const char* emx_path, const char* text_path, const char* nan_token);
+
 /*
  * CCM + CMX writer: Similarity::initialize / process / finish
  * (src/core/similarity.cpp:66-127,164-197; pairwise_matrix.cpp:51-57).

@@ -8,12 +8,19 @@
  * and satisfies min_corr <= |r| <= max_corr are kept and renumbered from 0;
  * a CCM record is written only when K > 1, a CMX record when K > 0; each
  * record's indent (x(x-1)/2 + y)*64 + cluster must exceed the previous one.
+ *
+ * The text importer/exporter is ImportExpressionMatrix::process
+ * (src/core/importexpressionmatrix.cpp:20-170) and
+ * ExportExpressionMatrix::process (src/core/exportexpressionmatrix.cpp:20-99),
+ * with the lines of a file parsed by several threads.
  */
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/kinc_formats.h"
@@ -163,6 +170,22 @@ struct PairwiseFile
       return KINC_OK;
    }
 
+   /* Records encoded elsewhere, in increasing indent order from first to
+    * last: Matrix::write's order check on the first, then one fwrite. */
+   int append(const std::string& bytes, int64_t records, int64_t first_indent, int64_t last_indent)
+   {
+      if ( records == 0 ) return KINC_OK;
+      if ( first_indent <= last_write )
+      {
+         return ferr(KINC_E_INVALID, "Attempting to write indent " + std::to_string(first_indent)
+                                     + " when last written is " + std::to_string(last_write));
+      }
+      if ( fwrite(bytes.data(), 1, bytes.size(), f) != bytes.size() ) return ferr(KINC_E_INVALID, path + ": write failed");
+      cluster_size += records;
+      last_write = last_indent;
+      return KINC_OK;
+   }
+
    /* Matrix::finish (pairwise_matrix.cpp:51-57): final counts, metadata */
    int close()
    {
@@ -193,8 +216,6 @@ struct kinc_similarity_writer
    int32_t max_clusters = 0;
    float min_corr = 0.5f;
    float max_corr = 1.0f;
-   std::vector<uint8_t> masks;
-   std::vector<float> kept;
 };
 
 extern "C" const char* kinc_formats_last_error(void)
@@ -324,75 +345,176 @@ extern "C" int kinc_similarity_writer_open(const char* ccm_path, const char* cmx
    return KINC_OK;
 }
 
+namespace {
+
+/* CCM nibble of a sample label for cluster k (Similarity::process,
+ * similarity.cpp:92-99): 1/0 for membership of a clustered sample, the
+ * negated marker (9 NaN, 8 post-outlier, 7 pre-outlier, 6 below minimum)
+ * otherwise, in 4 bits.  Two samples per byte, so the table maps a pair of
+ * labels (little-endian uint16) straight to the mask byte. */
+struct NibbleTable
+{
+   static constexpr int kTables = 8;   /* clusters 0..7; higher ones take the direct path */
+   uint8_t t[kTables][65536];
+   static uint8_t nib(int k, int8_t lab)
+   {
+      const int v = (lab >= 0) ? (k == lab ? 1 : 0) : -(int)lab;
+      return (uint8_t)(v & 0x0f);
+   }
+   NibbleTable()
+   {
+      for ( int k = 0; k < kTables; ++k )
+      {
+         for ( int u = 0; u < 65536; ++u )
+         {
+            t[k][u] = (uint8_t)(nib(k, (int8_t)(u & 0xff)) | (nib(k, (int8_t)(u >> 8)) << 4));
+         }
+      }
+   }
+};
+
+const NibbleTable& nibble_table()
+{
+   static const NibbleTable table;
+   return table;
+}
+
+/* The records of a run of pairs, encoded into memory. */
+struct Encoded
+{
+   std::string ccm, cmx;
+   int64_t ccm_recs = 0, ccm_pairs = 0, cmx_recs = 0, cmx_pairs = 0;
+   int64_t ccm_first = -1, ccm_last = -1, cmx_first = -1, cmx_last = -1;
+   int64_t bad = -1;        /* first pair that cannot be written */
+   std::string why;
+};
+
+inline void put_header(std::string& o, int32_t x, int32_t y, int c)
+{
+   char h[9];
+   for ( int i = 0; i < 4; ++i ) h[i] = (char)((uint32_t)x >> (8 * i));
+   for ( int i = 0; i < 4; ++i ) h[4 + i] = (char)((uint32_t)y >> (8 * i));
+   h[8] = (char)(int8_t)c;
+   o.append(h, 9);
+}
+
+} // namespace
+
+/*
+ * Similarity::process (similarity.cpp:66-127) for pairs [start, start+count).
+ * Threads encode contiguous runs of pairs into memory; the runs are then
+ * appended to the files in pair order, so the files are byte-identical to a
+ * serial pass.  A pair that cannot be written stops the call after every
+ * record before it has been written, as the serial loop would.
+ */
 extern "C" int kinc_similarity_writer_process(kinc_similarity_writer* w, int64_t start, int64_t count,
                                               const int8_t* K, const int8_t* labels,
                                               const float* corr, int32_t corr_stride)
 {
    if ( !w || !K || !corr || count < 0 ) return ferr(KINC_E_INVALID, "invalid process arguments");
+   if ( count == 0 ) return KINC_OK;
    const int S = w->S;
    const int nb = (S + 1) / 2;
-   int32_t x, y;
-   kinc_pair_index(start, &x, &y);
-   for ( int64_t q = 0; q < count; ++q )
-   {
-      const int k_sel = K[q];
-      const float* r = corr + q * corr_stride;
-      if ( k_sel > w->max_clusters )
+   const NibbleTable& lut = nibble_table();
+   const float lo = w->min_corr, hi = w->max_corr;
+   const int MC = w->max_clusters;
+
+   auto encode = [&](int64_t q0, int64_t q1, Encoded& e) {
+      int32_t x, y;
+      kinc_pair_index(start + q0, &x, &y);
+      e.ccm.reserve((size_t)(q1 - q0) * 16);
+      e.cmx.reserve((size_t)(q1 - q0) * 20);
+      for ( int64_t q = q0; q < q1; ++q )
       {
-         return ferr(KINC_E_INVALID, "Cannot write pair with cluster size exceeding the max");
-      }
-      if ( k_sel > 1 )
-      {
-         if ( !labels ) return ferr(KINC_E_INVALID, "labels are required when K > 1");
-         const int8_t* lab = labels + q * S;
-         w->masks.clear();
+         const int k_sel = K[q];
+         const float* r = corr + q * corr_stride;
+         if ( k_sel > MC )
+         {
+            e.bad = q;
+            e.why = "Cannot write pair with cluster size exceeding the max";
+            return;
+         }
+         if ( k_sel > 1 && !labels )
+         {
+            e.bad = q;
+            e.why = "labels are required when K > 1";
+            return;
+         }
+         const int64_t base_indent = ((int64_t)x * (x - 1) / 2 + y) * KINC_MAX_CLUSTER_SIZE;
          int n = 0;
          for ( int k = 0; k < k_sel; ++k )
          {
             const float c = r[k];
-            if ( !std::isnan(c) && w->min_corr <= std::fabs(c) && std::fabs(c) <= w->max_corr )
+            if ( std::isnan(c) || !(lo <= std::fabs(c) && std::fabs(c) <= hi) ) continue;
+            if ( k_sel > 1 )
             {
-               size_t base = w->masks.size();
-               w->masks.resize(base + (size_t)nb, 0);
-               for ( int i = 0; i < S; ++i )
+               const uint8_t* lab = (const uint8_t*)(labels + q * S);
+               put_header(e.ccm, x, y, n);
+               const size_t at = e.ccm.size();
+               e.ccm.resize(at + (size_t)nb);
+               uint8_t* m = (uint8_t*)&e.ccm[at];
+               const int pairs2 = S / 2;
+               if ( k < NibbleTable::kTables )
                {
-                  const int v = (lab[i] >= 0) ? (k == lab[i]) : -lab[i];
-                  const uint8_t nib = (uint8_t)(v & 0x0f);
-                  w->masks[base + (size_t)(i / 2)] |= (i & 1) ? (uint8_t)(nib << 4) : nib;
+                  const uint8_t* tk = lut.t[k];
+                  for ( int i = 0; i < pairs2; ++i ) m[i] = tk[lab[2 * i] | ((unsigned)lab[2 * i + 1] << 8)];
                }
-               ++n;
+               else
+               {
+                  for ( int i = 0; i < pairs2; ++i )
+                  {
+                     m[i] = (uint8_t)(NibbleTable::nib(k, (int8_t)lab[2 * i])
+                                      | (NibbleTable::nib(k, (int8_t)lab[2 * i + 1]) << 4));
+                  }
+               }
+               if ( S & 1 ) m[pairs2] = NibbleTable::nib(k, (int8_t)lab[S - 1]);
+               if ( e.ccm_first < 0 ) e.ccm_first = base_indent + n;
+               e.ccm_last = base_indent + n;
+               ++e.ccm_recs;
             }
+            put_header(e.cmx, x, y, n);
+            e.cmx.append((const char*)&c, 4);
+            if ( e.cmx_first < 0 ) e.cmx_first = base_indent + n;
+            e.cmx_last = base_indent + n;
+            ++e.cmx_recs;
+            ++n;
          }
-         for ( int c = 0; c < n; ++c )
+         if ( n > 0 )
          {
-            int rc = w->ccm.write(x, y, c, w->masks.data() + (size_t)c * (size_t)nb);
-            if ( rc ) return rc;
+            if ( k_sel > 1 ) ++e.ccm_pairs;
+            ++e.cmx_pairs;
          }
-         if ( n > 0 ) ++w->ccm.pair_size;
-      }
-      if ( k_sel > 0 )
-      {
-         int n = 0;
-         for ( int k = 0; k < k_sel; ++k )
+         /* Pairwise::Index::operator++ */
+         if ( ++y >= x )
          {
-            const float c = r[k];
-            if ( !std::isnan(c) && w->min_corr <= std::fabs(c) && std::fabs(c) <= w->max_corr )
-            {
-               uint8_t b[4];
-               memcpy(b, &c, 4);
-               int rc = w->cmx.write(x, y, n, b);
-               if ( rc ) return rc;
-               ++n;
-            }
+            y = 0;
+            ++x;
          }
-         if ( n > 0 ) ++w->cmx.pair_size;
       }
-      /* Pairwise::Index::operator++ */
-      if ( ++y >= x )
+   };
+
+   int T = (int)std::thread::hardware_concurrency();
+   T = std::max(1, std::min(T, 16));
+   T = (int)std::min<int64_t>(T, std::max<int64_t>(1, count / 2048));
+   std::vector<Encoded> parts((size_t)T);
+   if ( T == 1 ) encode(0, count, parts[0]);
+   else
+   {
+      std::vector<std::thread> pool;
+      for ( int t = 0; t < T; ++t )
       {
-         y = 0;
-         ++x;
+         pool.emplace_back(encode, count * t / T, count * (t + 1) / T, std::ref(parts[(size_t)t]));
       }
+      for ( auto& th : pool ) th.join();
+   }
+   for ( const Encoded& e : parts )
+   {
+      int rc = w->ccm.append(e.ccm, e.ccm_recs, e.ccm_first, e.ccm_last);
+      if ( rc == KINC_OK ) rc = w->cmx.append(e.cmx, e.cmx_recs, e.cmx_first, e.cmx_last);
+      if ( rc ) return rc;
+      w->ccm.pair_size += e.ccm_pairs;
+      w->cmx.pair_size += e.cmx_pairs;
+      if ( e.bad >= 0 ) return ferr(KINC_E_INVALID, e.why);
    }
    return KINC_OK;
 }
@@ -404,4 +526,606 @@ extern "C" int kinc_similarity_writer_close(kinc_similarity_writer* w)
    int rc2 = w->cmx.close();
    delete w;
    return rc1 ? rc1 : rc2;
+}
+
+/* ------------------------------------------------ expression matrix text */
+
+namespace {
+
+/* QRegExp("\\s+") word separators for ASCII text (QChar::isSpace). */
+inline bool is_space(char c)
+{
+   return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r';
+}
+
+struct Word
+{
+   const char* b;
+   const char* e;
+   std::string str() const { return std::string(b, e); }
+   bool equals(const std::string& t) const
+   {
+      return (size_t)(e - b) == t.size() && memcmp(b, t.data(), t.size()) == 0;
+   }
+};
+
+/* line.splitRef(QRegExp("\\s+"), QString::SkipEmptyParts) */
+void split_words(const char* b, const char* e, std::vector<Word>& out)
+{
+   out.clear();
+   const char* p = b;
+   while ( p < e )
+   {
+      while ( p < e && is_space(*p) ) ++p;
+      if ( p == e ) break;
+      const char* q = p;
+      while ( q < e && !is_space(*q) ) ++q;
+      out.push_back({p, q});
+      p = q;
+   }
+}
+
+const double kPow10[23] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10, 1e11,
+                           1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+
+/*
+ * A decimal number in C-locale syntax ([+-] digits [. digits] [(e|E) [+-]
+ * digits], at least one mantissa digit) or nan / inf / +inf / -inf, read to
+ * the nearest double.  Numbers whose digits fit in 2^53 with a decimal
+ * exponent of at most 22 are exact products/quotients of two doubles
+ * (Clinger's fast path); anything else goes to strtod, which rounds
+ * correctly too.  Returns false for anything else (hex, grouping, text).
+ */
+bool parse_double(const char* b, const char* e, double& out)
+{
+   const size_t n = (size_t)(e - b);
+   if ( n == 0 ) return false;
+   if ( (n == 3 && memcmp(b, "nan", 3) == 0) )
+   {
+      out = NAN;
+      return true;
+   }
+   if ( (n == 3 && memcmp(b, "inf", 3) == 0) || (n == 4 && memcmp(b, "+inf", 4) == 0) )
+   {
+      out = INFINITY;
+      return true;
+   }
+   if ( n == 4 && memcmp(b, "-inf", 4) == 0 )
+   {
+      out = -INFINITY;
+      return true;
+   }
+   const char* p = b;
+   bool neg = false;
+   if ( *p == '+' || *p == '-' )
+   {
+      neg = *p == '-';
+      ++p;
+   }
+   uint64_t mant = 0;
+   int digits = 0;     /* significant digits accumulated in mant */
+   int dropped = 0;    /* integer digits beyond 19 (scale up) */
+   int frac = 0;       /* fraction digits taken into mant */
+   bool any = false;
+   bool exact = true;
+   while ( p < e && *p >= '0' && *p <= '9' )
+   {
+      any = true;
+      if ( digits < 19 )
+      {
+         if ( mant != 0 || *p != '0' ) ++digits;
+         mant = mant * 10 + (uint64_t)(*p - '0');
+      }
+      else
+      {
+         ++dropped;
+         if ( *p != '0' ) exact = false;
+      }
+      ++p;
+   }
+   if ( p < e && *p == '.' )
+   {
+      ++p;
+      while ( p < e && *p >= '0' && *p <= '9' )
+      {
+         any = true;
+         if ( digits < 19 )
+         {
+            if ( mant != 0 || *p != '0' ) ++digits;
+            mant = mant * 10 + (uint64_t)(*p - '0');
+            ++frac;
+         }
+         else if ( *p != '0' )
+         {
+            exact = false;
+         }
+         ++p;
+      }
+   }
+   if ( !any ) return false;
+   int ex = 0;
+   if ( p < e && (*p == 'e' || *p == 'E') )
+   {
+      ++p;
+      bool eneg = false;
+      if ( p < e && (*p == '+' || *p == '-') )
+      {
+         eneg = *p == '-';
+         ++p;
+      }
+      if ( p == e ) return false;
+      while ( p < e && *p >= '0' && *p <= '9' )
+      {
+         if ( ex < 100000 ) ex = ex * 10 + (*p - '0');
+         ++p;
+      }
+      if ( eneg ) ex = -ex;
+   }
+   if ( p != e ) return false;
+   const int e10 = ex + dropped - frac;
+   if ( exact && mant <= (1ull << 53) && e10 >= -22 && e10 <= 22 )
+   {
+      double v = (double)mant;
+      v = (e10 >= 0) ? v * kPow10[e10] : v / kPow10[-e10];
+      out = neg ? -v : v;
+      return true;
+   }
+   std::string tmp(b, e);
+   out = strtod(tmp.c_str(), nullptr);
+   return true;
+}
+
+const char* kTransformNames[4] = {"none", "natural logarithm", "logarithm base 2", "logarithm base 10"};
+
+void json_string(std::string& o, const std::string& s)
+{
+   o.push_back('"');
+   for ( unsigned char c : s )
+   {
+      if ( c == '"' || c == '\\' )
+      {
+         o.push_back('\\');
+         o.push_back((char)c);
+      }
+      else if ( c < 0x20 )
+      {
+         char u[8];
+         snprintf(u, sizeof(u), "\\u%04x", c);
+         o += u;
+      }
+      else
+      {
+         o.push_back((char)c);
+      }
+   }
+   o.push_back('"');
+}
+
+void json_array(std::string& o, const std::vector<std::string>& v)
+{
+   o.push_back('[');
+   for ( size_t i = 0; i < v.size(); ++i )
+   {
+      if ( i ) o.push_back(',');
+      json_string(o, v[i]);
+   }
+   o.push_back(']');
+}
+
+/* Append code point cp as UTF-8. */
+void put_utf8(std::string& o, uint32_t cp)
+{
+   if ( cp < 0x80 ) o.push_back((char)cp);
+   else if ( cp < 0x800 )
+   {
+      o.push_back((char)(0xc0 | (cp >> 6)));
+      o.push_back((char)(0x80 | (cp & 0x3f)));
+   }
+   else if ( cp < 0x10000 )
+   {
+      o.push_back((char)(0xe0 | (cp >> 12)));
+      o.push_back((char)(0x80 | ((cp >> 6) & 0x3f)));
+      o.push_back((char)(0x80 | (cp & 0x3f)));
+   }
+   else
+   {
+      o.push_back((char)(0xf0 | (cp >> 18)));
+      o.push_back((char)(0x80 | ((cp >> 12) & 0x3f)));
+      o.push_back((char)(0x80 | ((cp >> 6) & 0x3f)));
+      o.push_back((char)(0x80 | (cp & 0x3f)));
+   }
+}
+
+/* A JSON string starting at s[i] == '"'; on success i is past its end. */
+bool json_read_string(const std::string& s, size_t& i, std::string& out)
+{
+   out.clear();
+   if ( i >= s.size() || s[i] != '"' ) return false;
+   ++i;
+   while ( i < s.size() )
+   {
+      char c = s[i++];
+      if ( c == '"' ) return true;
+      if ( c != '\\' )
+      {
+         out.push_back(c);
+         continue;
+      }
+      if ( i >= s.size() ) return false;
+      char x = s[i++];
+      switch ( x )
+      {
+      case '"': out.push_back('"'); break;
+      case '\\': out.push_back('\\'); break;
+      case '/': out.push_back('/'); break;
+      case 'b': out.push_back('\b'); break;
+      case 'f': out.push_back('\f'); break;
+      case 'n': out.push_back('\n'); break;
+      case 'r': out.push_back('\r'); break;
+      case 't': out.push_back('\t'); break;
+      case 'u':
+      {
+         if ( i + 4 > s.size() ) return false;
+         uint32_t cp = (uint32_t)strtoul(s.substr(i, 4).c_str(), nullptr, 16);
+         i += 4;
+         if ( cp >= 0xd800 && cp < 0xdc00 && i + 6 <= s.size() && s[i] == '\\' && s[i + 1] == 'u' )
+         {
+            uint32_t lo = (uint32_t)strtoul(s.substr(i + 2, 4).c_str(), nullptr, 16);
+            i += 6;
+            cp = 0x10000 + ((cp - 0xd800) << 10) + (lo - 0xdc00);
+         }
+         put_utf8(out, cp);
+         break;
+      }
+      default: return false;
+      }
+   }
+   return false;
+}
+
+void json_skip_ws(const std::string& s, size_t& i)
+{
+   while ( i < s.size() && (s[i] == ' ' || s[i] == '\t' || s[i] == '\n' || s[i] == '\r') ) ++i;
+}
+
+/* Skip one JSON value (string, number, literal, array or object). */
+bool json_skip_value(const std::string& s, size_t& i)
+{
+   json_skip_ws(s, i);
+   if ( i >= s.size() ) return false;
+   std::string tmp;
+   if ( s[i] == '"' ) return json_read_string(s, i, tmp);
+   if ( s[i] == '[' || s[i] == '{' )
+   {
+      int depth = 0;
+      while ( i < s.size() )
+      {
+         char c = s[i];
+         if ( c == '"' )
+         {
+            if ( !json_read_string(s, i, tmp) ) return false;
+            continue;
+         }
+         if ( c == '[' || c == '{' ) ++depth;
+         if ( c == ']' || c == '}' ) --depth;
+         ++i;
+         if ( depth == 0 ) return true;
+      }
+      return false;
+   }
+   while ( i < s.size() && s[i] != ',' && s[i] != '}' && s[i] != ']' ) ++i;
+   return true;
+}
+
+/* Value of a top-level key of a JSON object: an array of strings (found =
+ * true) or a string (as a one-element vector). */
+bool json_top_value(const std::string& s, const std::string& key, std::vector<std::string>& out)
+{
+   out.clear();
+   size_t i = 0;
+   json_skip_ws(s, i);
+   if ( i >= s.size() || s[i] != '{' ) return false;
+   ++i;
+   std::string k;
+   for ( ;; )
+   {
+      json_skip_ws(s, i);
+      if ( i >= s.size() || s[i] == '}' ) return false;
+      if ( !json_read_string(s, i, k) ) return false;
+      json_skip_ws(s, i);
+      if ( i >= s.size() || s[i] != ':' ) return false;
+      ++i;
+      json_skip_ws(s, i);
+      if ( k == key )
+      {
+         std::string v;
+         if ( i < s.size() && s[i] == '"' )
+         {
+            if ( !json_read_string(s, i, v) ) return false;
+            out.push_back(v);
+            return true;
+         }
+         if ( i >= s.size() || s[i] != '[' ) return false;
+         ++i;
+         for ( ;; )
+         {
+            json_skip_ws(s, i);
+            if ( i < s.size() && s[i] == ']' ) return true;
+            if ( i < s.size() && s[i] == '"' )
+            {
+               if ( !json_read_string(s, i, v) ) return false;
+               out.push_back(v);
+            }
+            else
+            {
+               /* a non-string element: keep its text */
+               size_t j = i;
+               if ( !json_skip_value(s, i) ) return false;
+               out.push_back(s.substr(j, i - j));
+            }
+            json_skip_ws(s, i);
+            if ( i < s.size() && s[i] == ',' ) ++i;
+         }
+      }
+      if ( !json_skip_value(s, i) ) return false;
+      json_skip_ws(s, i);
+      if ( i < s.size() && s[i] == ',' ) ++i;
+   }
+}
+
+int read_whole_file(const char* path, std::string& buf)
+{
+   FILE* f = fopen(path, "rb");
+   if ( !f ) return ferr(KINC_E_INVALID, std::string(path) + ": cannot open");
+   std::vector<char> chunk(1 << 20);
+   buf.clear();
+   size_t n;
+   while ( (n = fread(chunk.data(), 1, chunk.size(), f)) > 0 ) buf.append(chunk.data(), n);
+   const bool bad = ferror(f) != 0;
+   fclose(f);
+   if ( bad ) return ferr(KINC_E_INVALID, std::string(path) + ": read failed");
+   return KINC_OK;
+}
+
+struct ParseError
+{
+   int64_t line = -1;
+   std::string msg;
+};
+
+} // namespace
+
+extern "C" int kinc_emx_import_text(const char* text_path, const char* emx_path, const char* nan_token,
+                                    int32_t sample_size, int32_t transform, int32_t threads,
+                                    int32_t* gene_size_out, int32_t* sample_size_out)
+{
+   if ( !text_path || !emx_path || sample_size < 0 || transform < KINC_TRANSFORM_NONE
+        || transform > KINC_TRANSFORM_LOG10 )
+   {
+      return ferr(KINC_E_INVALID, "Did not get valid input and/or output arguments.");
+   }
+   std::string text;
+   int rc = read_whole_file(text_path, text);
+   if ( rc ) return rc;
+   const std::string token = nan_token ? nan_token : "";
+
+   /* QTextStream::readLine: lines end at '\n'; a final line without one
+    * still counts, an empty remainder after the last '\n' does not */
+   std::vector<std::pair<size_t, size_t>> lines;
+   {
+      size_t p = 0;
+      const size_t n = text.size();
+      while ( p < n )
+      {
+         const void* nl = memchr(text.data() + p, '\n', n - p);
+         size_t q = nl ? (size_t)((const char*)nl - text.data()) : n;
+         lines.push_back({p, q});
+         p = q + 1;
+      }
+   }
+
+   int S = sample_size;
+   std::vector<std::string> samples;
+   size_t first = 0;
+   std::vector<Word> words;
+   if ( S != 0 )
+   {
+      for ( int i = 0; i < S; ++i ) samples.push_back(std::to_string(i));
+   }
+   else
+   {
+      /* the first line(s) until one has words: the sample names */
+      while ( first < lines.size() && S == 0 )
+      {
+         split_words(text.data() + lines[first].first, text.data() + lines[first].second, words);
+         S = (int)words.size();
+         for ( const Word& w : words ) samples.push_back(w.str());
+         ++first;
+      }
+   }
+   const int64_t G64 = (int64_t)(lines.size() - first);
+   if ( S < 1 || G64 < 1 ) return ferr(KINC_E_INVALID, std::string(text_path) + ": no expression data");
+   if ( G64 > INT32_MAX ) return ferr(KINC_E_INVALID, std::string(text_path) + ": too many genes");
+   const int G = (int)G64;
+
+   std::vector<float> data((size_t)G * (size_t)S);
+   std::vector<std::string> genes((size_t)G);
+
+   int T = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+   T = std::max(1, std::min(T, 16));
+   T = std::min(T, std::max(1, G / 256));
+   std::vector<ParseError> errs((size_t)T);
+
+   auto work = [&](int t) {
+      const int g0 = (int)((int64_t)G * t / T), g1 = (int)((int64_t)G * (t + 1) / T);
+      std::vector<Word> w;
+      for ( int g = g0; g < g1; ++g )
+      {
+         const auto& ln = lines[first + (size_t)g];
+         split_words(text.data() + ln.first, text.data() + ln.second, w);
+         if ( (int)w.size() != S + 1 )
+         {
+            errs[t].line = (int64_t)g;
+            errs[t].msg = "Encountered gene expression line with incorrect amount of fields. Read in "
+                          + std::to_string((long long)w.size() - 1) + " fields when it should have been "
+                          + std::to_string(S) + ". Gene name is " + (w.empty() ? std::string() : w[0].str())
+                          + ".";
+            return;
+         }
+         genes[(size_t)g] = w[0].str();
+         float* row = data.data() + (size_t)g * (size_t)S;
+         for ( int i = 1; i <= S; ++i )
+         {
+            if ( !token.empty() && w[i].equals(token) )
+            {
+               row[i - 1] = NAN;
+               continue;
+            }
+            double v;
+            if ( !parse_double(w[i].b, w[i].e, v) )
+            {
+               errs[t].line = (int64_t)g;
+               errs[t].msg = "Failed to read expression value \"" + w[i].str() + "\" for gene "
+                             + w[0].str() + ".";
+               return;
+            }
+            switch ( transform )
+            {
+            case KINC_TRANSFORM_NLOG: v = std::log(v); break;
+            case KINC_TRANSFORM_LOG2: v = std::log2(v); break;
+            case KINC_TRANSFORM_LOG10: v = std::log10(v); break;
+            default: break;
+            }
+            row[i - 1] = (float)v;
+         }
+      }
+   };
+   if ( T == 1 ) work(0);
+   else
+   {
+      std::vector<std::thread> pool;
+      for ( int t = 0; t < T; ++t ) pool.emplace_back(work, t);
+      for ( auto& th : pool ) th.join();
+   }
+   /* the reference stops at the first bad line in file order */
+   for ( const ParseError& e : errs )
+   {
+      if ( e.line >= 0 ) return ferr(KINC_E_INVALID, e.msg);
+   }
+
+   std::string meta = "{\"genes\": ";
+   json_array(meta, genes);
+   meta += ", \"samples\": ";
+   json_array(meta, samples);
+   meta += ", \"transform\": ";
+   json_string(meta, kTransformNames[transform]);
+   meta += "}";
+   rc = kinc_emx_write(emx_path, data.data(), G, S, meta.c_str());
+   if ( rc ) return rc;
+   if ( gene_size_out ) *gene_size_out = G;
+   if ( sample_size_out ) *sample_size_out = S;
+   return KINC_OK;
+}
+
+extern "C" int kinc_emx_export_text(const char* emx_path, const char* text_path, const char* nan_token)
+{
+   if ( !emx_path || !text_path ) return ferr(KINC_E_INVALID, "Did not get valid input and/or output arguments.");
+   int32_t G = 0, S = 0;
+   int rc = kinc_emx_info(emx_path, &G, &S);
+   if ( rc ) return rc;
+   std::vector<float> data((size_t)G * (size_t)S);
+   rc = kinc_emx_read(emx_path, data.data());
+   if ( rc ) return rc;
+   int64_t msize = kinc_file_metadata(emx_path, nullptr, 0);
+   if ( msize < 0 ) return (int)msize;
+   std::string meta((size_t)msize, '\0');
+   if ( msize > 0 && kinc_file_metadata(emx_path, &meta[0], msize) < 0 ) return KINC_E_INVALID;
+
+   std::vector<std::string> genes, samples, tname;
+   if ( !json_top_value(meta, "genes", genes) || (int)genes.size() != G )
+   {
+      genes.clear();
+      for ( int i = 0; i < G; ++i ) genes.push_back(std::to_string(i));
+   }
+   if ( !json_top_value(meta, "samples", samples) || (int)samples.size() != S )
+   {
+      samples.clear();
+      for ( int i = 0; i < S; ++i ) samples.push_back(std::to_string(i));
+   }
+   int transform = KINC_TRANSFORM_NONE;
+   if ( json_top_value(meta, "transform", tname) && tname.size() == 1 )
+   {
+      for ( int t = 0; t < 4; ++t )
+      {
+         if ( tname[0] == kTransformNames[t] ) transform = t;
+      }
+   }
+   const std::string token = nan_token ? nan_token : "";
+
+   /* rows are formatted by several threads into per-thread buffers of
+    * about 8 MB, which are written in gene order */
+   int T = (int)std::thread::hardware_concurrency();
+   T = std::max(1, std::min(T, 16));
+   const int rows_per_chunk = std::max(1, (int)((8u << 20) / ((size_t)S * 14 + 16)));
+   auto format_rows = [&](int g0, int g1, std::string& out) {
+      char num[64];
+      out.clear();
+      for ( int g = g0; g < g1; ++g )
+      {
+         out += genes[(size_t)g];
+         const float* row = data.data() + (size_t)g * (size_t)S;
+         for ( int j = 0; j < S; ++j )
+         {
+            float value = row[j];
+            out.push_back('\t');
+            if ( std::isnan(value) )
+            {
+               out += token;
+               continue;
+            }
+            switch ( transform )
+            {
+            case KINC_TRANSFORM_NLOG: value = std::exp(value); break;
+            case KINC_TRANSFORM_LOG2: value = (float)std::pow(2.0, (double)value); break;
+            case KINC_TRANSFORM_LOG10: value = (float)std::pow(10.0, (double)value); break;
+            default: break;
+            }
+            snprintf(num, sizeof(num), "%.12g", (double)value);
+            out += num;
+         }
+         out.push_back('\n');
+      }
+   };
+
+   FILE* f = fopen(text_path, "wb");
+   if ( !f ) return ferr(KINC_E_INVALID, std::string(text_path) + ": cannot open for writing");
+   std::string head;
+   for ( int i = 0; i < S; ++i )
+   {
+      head += samples[(size_t)i];
+      head.push_back('\t');
+   }
+   head.push_back('\n');
+   bool ok = fwrite(head.data(), 1, head.size(), f) == head.size();
+   std::vector<std::string> bufs((size_t)T);
+   for ( int g = 0; g < G && ok; g += T * rows_per_chunk )
+   {
+      std::vector<std::thread> pool;
+      int used = 0;
+      for ( int t = 0; t < T; ++t )
+      {
+         const int g0 = g + t * rows_per_chunk;
+         if ( g0 >= G ) break;
+         const int g1 = std::min(G, g0 + rows_per_chunk);
+         pool.emplace_back(format_rows, g0, g1, std::ref(bufs[(size_t)t]));
+         ++used;
+      }
+      for ( auto& th : pool ) th.join();
+      for ( int t = 0; t < used && ok; ++t )
+      {
+         ok = fwrite(bufs[(size_t)t].data(), 1, bufs[(size_t)t].size(), f) == bufs[(size_t)t].size();
+      }
+   }
+   ok = (fclose(f) == 0) && ok;
+   return ok ? KINC_OK : ferr(KINC_E_INVALID, std::string(text_path) + ": write failed");
 }

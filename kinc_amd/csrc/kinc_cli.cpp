@@ -9,7 +9,12 @@
  *        [--minexpr F] [--minsamp N] [--minclus N] [--maxclus N]
  *        [--crit BIC|ICL] [--preout true|false] [--postout true|false]
  *        [--mincorr F] [--maxcorr F] [--ksize N]
- * plus this build's --device N and --form plain|persistent.
+ * plus this build's --device N and --form plain|persistent; and the
+ * expression-matrix converters around it (importexpressionmatrix_input.cpp,
+ * exportexpressionmatrix_input.cpp):
+ *   kinc run import-emx --input A.txt --output A.emx [--nan TOKEN]
+ *        [--samples N] [--transform none|"natural logarithm"|...] [--threads N]
+ *   kinc run export-emx --input A.emx --output A.txt [--nan TOKEN]
  *
  * The ACE driver loop is Similarity::size/makeWork (similarity.cpp:17-39)
  * then execute and process in pair order.  Here the device executes block
@@ -42,8 +47,67 @@ int usage()
            "usage: kinc run similarity --input EMX --ccm CCM --cmx CMX [--clusmethod none|gmm|kmeans]\n"
            "       [--corrmethod pearson|spearman] [--minexpr F] [--minsamp N] [--minclus N]\n"
            "       [--maxclus N] [--crit BIC|ICL] [--preout BOOL] [--postout BOOL]\n"
-           "       [--mincorr F] [--maxcorr F] [--ksize N] [--device N] [--form plain|persistent]\n");
+           "       [--mincorr F] [--maxcorr F] [--ksize N] [--device N] [--form plain|persistent]\n"
+           "       kinc run import-emx --input TXT --output EMX [--nan TOKEN] [--samples N]\n"
+           "       [--transform none|'natural logarithm'|'logarithm base 2'|'logarithm base 10']\n"
+           "       [--threads N]\n"
+           "       kinc run export-emx --input EMX --output TXT [--nan TOKEN]\n");
    return 2;
+}
+
+/* import-emx / export-emx */
+int run_convert(bool import, int argc, char** argv)
+{
+   static const char* kNames[4] = {"none", "natural logarithm", "logarithm base 2", "logarithm base 10"};
+   std::string in, out, nan;
+   int samples = 0, transform = KINC_TRANSFORM_NONE, threads = 0;
+   for ( int i = 3; i < argc; i += 2 )
+   {
+      if ( i + 1 >= argc ) return usage();
+      const std::string k = argv[i], v = argv[i + 1];
+      if ( k == "--input" ) in = v;
+      else if ( k == "--output" ) out = v;
+      else if ( k == "--nan" ) nan = v;
+      else if ( import && k == "--samples" ) samples = atoi(v.c_str());
+      else if ( import && k == "--threads" ) threads = atoi(v.c_str());
+      else if ( import && k == "--transform" )
+      {
+         transform = -1;
+         for ( int t = 0; t < 4; ++t )
+         {
+            if ( v == kNames[t] ) transform = t;
+         }
+         if ( transform < 0 ) return usage();
+      }
+      else return usage();
+   }
+   if ( in.empty() || out.empty() )
+   {
+      fprintf(stderr, "Did not get valid input and/or output arguments.\n");
+      return 2;
+   }
+   int rc;
+   if ( import )
+   {
+      int32_t G = 0, S = 0;
+      auto t0 = std::chrono::steady_clock::now();
+      rc = kinc_emx_import_text(in.c_str(), out.c_str(), nan.c_str(), samples, transform, threads, &G, &S);
+      if ( rc == KINC_OK )
+      {
+         double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+         fprintf(stderr, "imported %d genes x %d samples in %.2f s\n", G, S, s);
+      }
+   }
+   else
+   {
+      rc = kinc_emx_export_text(in.c_str(), out.c_str(), nan.c_str());
+   }
+   if ( rc != KINC_OK )
+   {
+      fprintf(stderr, "%s\n", kinc_formats_last_error());
+      return 1;
+   }
+   return 0;
 }
 
 struct HostSet
@@ -64,7 +128,10 @@ struct DevSet
 
 int main(int argc, char** argv)
 {
-   if ( argc < 3 || std::string(argv[1]) != "run" || std::string(argv[2]) != "similarity" ) return usage();
+   if ( argc < 3 || std::string(argv[1]) != "run" ) return usage();
+   if ( std::string(argv[2]) == "import-emx" ) return run_convert(true, argc, argv);
+   if ( std::string(argv[2]) == "export-emx" ) return run_convert(false, argc, argv);
+   if ( std::string(argv[2]) != "similarity" ) return usage();
    kinc_sim_params p;
    kinc_sim_default_params(&p);
    std::string in, ccm, cmx;

@@ -32,7 +32,13 @@ SIGNATURES = {
                                                       _P, ctypes.c_int32]),
     "kinc_similarity_writer_close": (ctypes.c_int, [_P]),
     "kinc_formats_last_error": (ctypes.c_char_p, []),
+    "kinc_emx_import_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            _P, _P]),
+    "kinc_emx_export_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
 }
+# ExpressionMatrix::TRANSFORM_NAMES (src/core/expressionmatrix.cpp:8-14)
+TRANSFORMS = ["none", "natural logarithm", "logarithm base 2", "logarithm base 10"]
 _bound = False
 
 
@@ -62,6 +68,23 @@ def write_emx(path, data, genes=None, samples=None, transform="none"):
             "transform": transform}
     _check(_bind().kinc_emx_write(path.encode(), data.ctypes.data, G, S,
                                   json.dumps(meta).encode()))
+
+
+def import_text(text_path, emx_path, nan=None, samples=0, transform="none", threads=0):
+    """ImportExpressionMatrix: whitespace-delimited text to an EMX file
+    (kinc_emx_import_text).  Returns (genes, samples) sizes."""
+    G = ctypes.c_int32()
+    S = ctypes.c_int32()
+    _check(_bind().kinc_emx_import_text(
+        text_path.encode(), emx_path.encode(), nan.encode() if nan else None, samples,
+        TRANSFORMS.index(transform), threads, ctypes.byref(G), ctypes.byref(S)))
+    return G.value, S.value
+
+
+def export_text(emx_path, text_path, nan=""):
+    """ExportExpressionMatrix: an EMX file to text (kinc_emx_export_text)."""
+    _check(_bind().kinc_emx_export_text(emx_path.encode(), text_path.encode(),
+                                        (nan or "").encode()))
 
 
 def read_emx_native(path):

@@ -116,3 +116,65 @@ def test_writer_refuses_out_of_order_blocks(tmp_path):
 def test_writer_rejects_bad_max_clusters(tmp_path):
     with pytest.raises(_native.KincError):
         formats.SimilarityWriter(str(tmp_path / "a"), str(tmp_path / "b"), 5, 4, 65)
+
+
+def _check_files(ccm, cmx, want_ccm, want_cmx, S):
+    hdr, rec = formats.read_pairwise(ccm)
+    assert hdr["clusters"] == len(want_ccm)
+    assert hdr["pairs"] == len({(a, b) for a, b, _, _ in want_ccm})
+    masks = formats.ccm_masks(rec, S)
+    got = list(zip(rec["x"].tolist(), rec["y"].tolist(), rec["cluster"].tolist()))
+    assert got == [(x, y, c) for x, y, c, _ in want_ccm]
+    assert masks.tolist() == [m for _, _, _, m in want_ccm]
+    hdr, rec = formats.read_pairwise(cmx)
+    assert hdr["clusters"] == len(want_cmx)
+    assert hdr["pairs"] == len({(a, b) for a, b, _, _ in want_cmx})
+    got = list(zip(rec["x"].tolist(), rec["y"].tolist(), rec["cluster"].tolist()))
+    assert got == [(x, y, c) for x, y, c, _ in want_cmx]
+    assert formats.cmx_values(rec).view(np.uint32).tolist() == \
+        np.array([v for _, _, _, v in want_cmx], np.float32).view(np.uint32).tolist()
+
+
+def _random_block(rng, n, S, MC):
+    K = rng.integers(0, MC + 1, n).astype(np.int8)
+    labels = rng.choice(np.array([-9, -8, -7, -6, -1, 0, 1, 2, 3, 4, 5, 6, 7], np.int8), (n, S))
+    corr = rng.uniform(-1, 1, (n, MC)).astype(np.float32)
+    corr[rng.random(corr.shape) < 0.1] = np.nan
+    return K, labels, corr
+
+
+@pytest.mark.parametrize("S", [7, 8])
+def test_writer_large_block_matches_model(tmp_path, S):
+    # large enough that the records are encoded by several threads; an odd
+    # sample count leaves a half-filled last mask byte
+    rng = np.random.default_rng(S)
+    G, MC = 130, 8
+    n = G * (G - 1) // 2
+    K, labels, corr = _random_block(rng, n, S, MC)
+    ccm, cmx = str(tmp_path / "l.ccm"), str(tmp_path / "l.cmx")
+    w = formats.SimilarityWriter(ccm, cmx, G, S, MC, min_corr=0.4, max_corr=0.95)
+    w.process(0, K, labels, corr)
+    w.close()
+    want_ccm, want_cmx = _model_process(0, K, labels, corr, S, np.float32(0.4), np.float32(0.95))
+    _check_files(ccm, cmx, want_ccm, want_cmx, S)
+
+
+def test_writer_stops_at_the_first_bad_pair(tmp_path):
+    # a pair with K > max_clusters: every record before it is written, none after
+    rng = np.random.default_rng(9)
+    G, S, MC = 130, 6, 4
+    n = G * (G - 1) // 2
+    K, labels, corr = _random_block(rng, n, S, MC)
+    labels = np.where(labels > 3, 0, labels).astype(np.int8)
+    bad = 5000
+    K[bad] = MC + 1
+    K[bad + 300] = MC + 2
+    ccm, cmx = str(tmp_path / "b.ccm"), str(tmp_path / "b.cmx")
+    w = formats.SimilarityWriter(ccm, cmx, G, S, MC, min_corr=0.2)
+    with pytest.raises(_native.KincError) as e:
+        w.process(0, K, labels, corr)
+    assert "cluster size exceeding the max" in str(e.value)
+    w.close()
+    want_ccm, want_cmx = _model_process(0, K[:bad], labels[:bad], corr[:bad], S,
+                                        np.float32(0.2), 1.0)
+    _check_files(ccm, cmx, want_ccm, want_cmx, S)
