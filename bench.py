@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "gene pairs/sec GMM+Spearman 20k×500, batched vs plain; p50/p99 query latency"
 FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "H": "KINC similarity, GMM (K=1..5, BIC) + Spearman, pre/post outliers, minsamp 30; "
          "20000 genes x 500 samples FP32, module generator, 10% NaN",
@@ -368,9 +369,25 @@ def main():
             flops_by_P[int(P)] = flops_by_P.get(int(P), 0.0) + float(f[Ps == P].sum())
     # the dominant kernel is the bin instance that carries the most work; bins
     # run concurrently, so each one's duration spans the others' too
-    dom = max((P for P in flops_by_P if P in prof), key=lambda P: flops_by_P[P])
-    launches, dom_ms = prof[dom]
-    achieved = flops_by_P.get(dom, 0.0) / launches / (dom_ms / launches * 1e-3) / 1e12
+    clustered = flags.get("clusmethod", "none") != "none"
+    if clustered:
+        dom = max((P for P in flops_by_P if P in prof), key=lambda P: flops_by_P[P])
+        launches, dom_ms = prof[dom]
+        achieved = flops_by_P.get(dom, 0.0) / launches / (dom_ms / launches * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                "flops_formula": "GMM: N*K*(18T + 20T_M) + 2NT per fit; K-means: N*T*(6K + 2); "
+                                 "Pearson: 8 per in-cluster point"}
+    else:
+        # without clustering one launch strides over the block: HBM-bound
+        # (algorithmic bytes per pair 8*S + 1 + 4*maxclus)
+        dom = max(prof, key=lambda P: prof[P][1])
+        launches, dom_ms = prof[dom]
+        per_pair = 8 * S + 1 + 4 * flags.get("maxclus", 5)
+        achieved = per_pair * pairs / (dom_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "bytes_formula": "8*S + 1 + S*[K>1] + 4*maxclus per pair"}
     whole_tflops = flops_total / secs / 1e12
 
     cpu = None
@@ -425,19 +442,12 @@ def main():
         },
         "plain_value": plain_value,
         "batched_vs_plain": (value / plain_value) if plain_value else None,
-        "roofline": {
-            "bound": "valu",
+        "roofline": dict(roof, **{
             "kernel": "k_pairs<P=%d,KMAX=5> (persistent form)" % dom,
-            "flops_formula": "GMM: N*K*(18T + 20T_M) + 2NT per fit; K-means: N*T*(6K + 2); "
-                             "Pearson: 8 per in-cluster point",
-            "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-            "traffic_source": TRAFFIC_SOURCE,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH if args.config == "H" else None,
+            "traffic_source": TRAFFIC_SOURCE if args.config == "H" else None,
             "whole_job_tflops": whole_tflops,
-        },
+        }),
         "cpu_baseline": cpu,
         "serving": serving_obj,
     }

@@ -475,7 +475,8 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
    a.corr_out = corr;
    a.iters_out = iters;
 
-   const bool need_counts = (h->p.form == KINC_FORM_PERSISTENT) || (h->p.corr_method == KINC_CORR_PEARSON);
+   const bool binned = h->p.form == KINC_FORM_PERSISTENT && h->p.clus_method != KINC_CLUS_NONE;
+   const bool need_counts = binned || (h->p.corr_method == KINC_CORR_PEARSON);
    if ( need_counts )
    {
       const float* emx = h->d_emx;
@@ -517,6 +518,25 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
          if ( rc ) return rc;
       }
       return KINC_OK;
+   }
+
+   if ( h->p.clus_method == KINC_CLUS_NONE )
+   {
+      /* no fits: the slot count does not matter and every pair costs about
+       * the same, so resident waves stride over the whole block in one launch */
+      const int P = h->classes[0];
+      const void* fn = kinc_select_pair_kernel(P, h->kmax);
+      if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
+      int per_cu = 0;
+      HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
+      if ( per_cu < 1 ) per_cu = 1;
+      long long waves = (long long)per_cu * h->num_cus;
+      if ( P == 0 ) waves = std::min<long long>(waves, h->scratch_slots);
+      a.list = nullptr;
+      a.first = 0;
+      a.launch_count = cnt;
+      void* args[] = {&a};
+      return launch_pairs(h, P, fn, (unsigned)std::min<long long>(waves, (long long)cnt), args, s);
    }
 
    /* persistent: bin by slot count, then drain each bin with resident waves */

@@ -351,13 +351,23 @@ __device__ void copy_back(const Lds& L, int N)
 
 /* --------------------------------------------------------- chain helpers */
 
-/* Add row `lane` of the staged tile (len values) to acc in index order. */
+/*
+ * A wave64 VALU instruction with 8 or fewer active lanes issues about 5x
+ * slower on gfx950 than one with 9 or more (measured: 0.19 vs 0.98 wave
+ * instructions per CU cycle for a v_add_f32 stream).  Loops that only a few
+ * lanes need therefore run on at least this many lanes, the extra ones
+ * computing values that are never read.
+ */
+#define KINC_MIN_ACTIVE_LANES 16
+
+/* Add row `lane` of the staged tile (len values) to acc in index order, for
+ * lanes < nch; lanes nch..15 repeat the last row and their sums are unused. */
 __device__ __forceinline__ float chain_add(const float* tile, int stride, int nch, int len, float acc)
 {
    const int lane = lane_id();
-   if ( lane < nch )
+   if ( lane < nch || lane < KINC_MIN_ACTIVE_LANES )
    {
-      const float* row = tile + lane * stride;
+      const float* row = tile + (lane < nch ? lane : nch - 1) * stride;
       if ( len == TILE )
       {
          const float4* r4 = (const float4*)row;
@@ -540,12 +550,16 @@ __device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm)
          float lgm[K];
 #pragma unroll
          for ( int k = 0; k < K; ++k ) lgm[k] = st.LG(k, m);
+         /* fmaxf(mx, v) is (v > mx) ? v : mx here: mx is never NaN (a NaN v
+          * leaves it alone either way), and when the two differ only in the
+          * sign of a zero maximum, every use below (v - mx, mx + log(sum))
+          * gives the same value */
          float mx = -INFINITY;
 #pragma unroll
          for ( int k = 0; k < K; ++k )
          {
             float v = logpi[k] + lgm[k];
-            mx = (v > mx) ? v : mx;
+            mx = fmaxf(mx, v);
          }
          float sum = 0.0f;
 #pragma unroll
@@ -608,6 +622,10 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       gmm_estep<ST, K>(nslots, st, prm);
       e_steps++;
 
+      /* max over the points of each component's log responsibility; only
+       * the last slot holds lanes past N.  fmaxf as in the E step: v is never
+       * NaN, and the sign of a zero maximum does not reach l - gmax or
+       * gmax + log(sum) */
       float gmax[K];
 #pragma unroll
       for ( int k = 0; k < K; ++k )
@@ -616,11 +634,14 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
 #pragma unroll
          for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
          {
-            if ( m < nslots )
+            if ( m < nslots - 1 )
             {
-               const bool valid = lane + m * WAVE < N;
+               v = fmaxf(v, st.LG(k, m));
+            }
+            else if ( m == nslots - 1 )
+            {
                const float l = st.LG(k, m);
-               v = (valid && l > v) ? l : v;
+               v = (lane + m * WAVE < N) ? fmaxf(v, l) : v;
             }
          }
          gmax[k] = wave_max(v);
@@ -972,7 +993,7 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
       }
       wave_sync(L.lgbuf != nullptr);
       float Ssum = 0.0f;
-      if ( lane == 0 )
+      if ( lane < KINC_MIN_ACTIVE_LANES )   /* identical sums; lane 0's is used */
       {
          for ( int j = 0; j < kbase; ++j ) Ssum = Ssum + seq[j];
       }
@@ -1112,7 +1133,8 @@ __device__ void pearson_all(const SimArgs& a, const Lds& L, int K, float* corr)
    const int lane = lane_id();
    const int nch = 5 * K;
    float acc = 0.0f;
-   if ( lane < nch )
+   /* lanes nch..15 (cluster index >= K, which no label takes) add nothing */
+   if ( lane < nch || lane < KINC_MIN_ACTIVE_LANES )
    {
       const int k = lane / 5;
       const int t = lane - 5 * k;
@@ -1369,22 +1391,24 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
 /* ------------------------------------------------------------- kernels */
 
 /*
- * The pair kernel of both execution forms.  Plain form (a.list == null): one
- * workgroup per pair of the launch.  Persistent form: resident waves pop
- * block-relative pair ids from a device queue until it is empty, so every
- * wave reaches the exit test.
+ * The pair kernel of both execution forms.  Without a list (a.list == null)
+ * the workgroups stride over the launch's pairs: the plain form's one
+ * workgroup per pair, or resident waves over the whole block (persistent
+ * form without clustering).  With a list, resident waves pop block-relative
+ * pair ids from a device queue until it is empty, so every wave reaches the
+ * exit test.
  */
 template <int P, int KMAX>
 __global__ KINC_PAIR_BOUNDS void k_pairs(SimArgs a)
 {
    if ( a.list == nullptr )
    {
-      /* the memory-store instance owns one scratch slice per workgroup, so
-       * its grid is capped and strides over the launch */
+      /* workgroups stride over the launch: the grid is one workgroup per
+       * pair, or capped (the memory-store instance's scratch slices, the
+       * persistent form without clustering) */
       for ( int q = a.first + (int)blockIdx.x; q < a.first + a.launch_count; q += (int)gridDim.x )
       {
          process_pair<P, KMAX>(a, q);
-         if ( P > 0 ) break;
       }
       return;
    }
