@@ -46,6 +46,15 @@ def lib():
             "oracle_corr_fp64_block": (ctypes.c_int, [P, P, i32, i32, i64, i64, P, P, P]),
             "oracle_math_array": (None, [ctypes.c_int, P, P, i64]),
             "oracle_execute_pairs": (ctypes.c_int, [P, P, i32, i32, P, P, i64, P, P, P, P]),
+            "oracle_rmt_maximums": (None, [P, ctypes.c_int, ctypes.c_int, P]),
+            "oracle_rmt_prune": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, f32, P, P]),
+            "oracle_rmt_eigenvalues": (None, [P, ctypes.c_int, P]),
+            "oracle_akima": (ctypes.c_int, [P, P, ctypes.c_int, P, ctypes.c_int, P]),
+            "oracle_rmt_chi_square": (f32, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int]),
+            "oracle_rmt_run": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, f32, f32, f32, f32,
+                                              f32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, P, P, P, ctypes.c_int, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -186,3 +195,68 @@ def execute_pairs(params, emx, pairs, seg_lo=None, want_labels=True):
                                _ptr(seg), count, K.ctypes.data, _ptr(labels), corr.ctypes.data,
                                iters.ctypes.data)
     return K, labels, corr, iters
+
+
+# ------------------------------------------------------------------ RMT
+
+def rmt_maximums(matrix):
+    """RMT::computeMaximums on a dense [N][N][K] float32 matrix."""
+    m = np.ascontiguousarray(matrix, np.float32)
+    N, _, K = m.shape
+    out = np.empty(N * K, np.float32)
+    lib().oracle_rmt_maximums(m.ctypes.data, N, K, out.ctypes.data)
+    return out
+
+
+def rmt_prune(matrix, maximums, threshold):
+    m = np.ascontiguousarray(matrix, np.float32)
+    N, _, K = m.shape
+    mx = np.ascontiguousarray(maximums, np.float32)
+    n = int((mx >= np.float32(threshold)).sum())
+    idx = np.empty(max(n, 1), np.int32)
+    pr = np.empty((max(n, 1), max(n, 1)), np.float32)
+    got = lib().oracle_rmt_prune(m.ctypes.data, mx.ctypes.data, N, K, threshold, idx.ctypes.data,
+                                 pr.ctypes.data)
+    assert got == n
+    return idx[:n], pr[:n, :n]
+
+
+def rmt_eigenvalues(prune):
+    p = np.ascontiguousarray(prune, np.float32)
+    n = p.shape[0]
+    out = np.empty(n, np.float32)
+    lib().oracle_rmt_eigenvalues(p.ctypes.data, n, out.ctypes.data)
+    return out
+
+
+def akima(x, y, xe):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    xe = np.ascontiguousarray(xe, np.float64)
+    ye = np.empty_like(xe)
+    rc = lib().oracle_akima(x.ctypes.data, y.ctypes.data, x.size, xe.ctypes.data, xe.size,
+                            ye.ctypes.data)
+    if rc:
+        raise ValueError("akima init failed: %d" % rc)
+    return ye
+
+
+def rmt_chi_square(eigens, min_eigen=50, min_pace=10, max_pace=40, bins=60):
+    e = np.ascontiguousarray(eigens, np.float32)
+    return lib().oracle_rmt_chi_square(e.ctypes.data, e.size, min_eigen, min_pace, max_pace, bins)
+
+
+def rmt_run(matrix, start=0.99, step=0.001, stop=0.5, chi1=99.607, chi2=200.0, min_eigen=50,
+            min_pace=10, max_pace=40, bins=60, cap=1000):
+    """RMT::process; returns (rc_or_steps, final_threshold, log rows (t, size, chi))."""
+    m = np.ascontiguousarray(matrix, np.float32)
+    N, _, K = m.shape
+    lt = np.empty(cap, np.float32)
+    ls = np.empty(cap, np.int32)
+    lc = np.empty(cap, np.float32)
+    ft = ctypes.c_float()
+    rc = lib().oracle_rmt_run(m.ctypes.data, N, K, start, step, stop, chi1, chi2, min_eigen,
+                              min_pace, max_pace, bins, lt.ctypes.data, ls.ctypes.data,
+                              lc.ctypes.data, cap, ctypes.byref(ft))
+    n = min(rc, cap) if rc > 0 else min(cap, int(round((start - stop) / step)) + 1)
+    return rc, ft.value, list(zip(lt[:n].tolist(), ls[:n].tolist(), lc[:n].tolist()))
