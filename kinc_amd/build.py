@@ -41,6 +41,7 @@ SOURCES = [
     "kinc_sim.cpp",
     "formats.cpp",
     "server.cpp",
+    "rmt.hip",
 ]
 # pair-kernel instances (KMAX, P): register-slot counts P, and P = 0 for the
 # memory-store instance used above the register range (1024 samples for up to
@@ -70,7 +71,9 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + [
+        "-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",
+        "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))

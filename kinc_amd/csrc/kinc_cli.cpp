@@ -15,6 +15,9 @@
  *   kinc run import-emx --input A.txt --output A.emx [--nan TOKEN]
  *        [--samples N] [--transform none|"natural logarithm"|...] [--threads N]
  *   kinc run export-emx --input A.emx --output A.txt [--nan TOKEN]
+ * and the RMT threshold analytic after it (rmt_input.cpp:58-136):
+ *   kinc run rmt --input A.cmx --log A.log [--tstart F] [--tstep F]
+ *        [--tstop F] [--minpace N] [--maxpace N] [--bins N] [--device N]
  *
  * The ACE driver loop is Similarity::size/makeWork (similarity.cpp:17-39)
  * then execute and process in pair order.  Here the device executes block
@@ -32,6 +35,7 @@
 #include <vector>
 
 #include "../../include/kinc_formats.h"
+#include "../../include/kinc_rmt.h"
 #include "../../include/kinc_sim.h"
 
 namespace {
@@ -51,8 +55,48 @@ int usage()
            "       kinc run import-emx --input TXT --output EMX [--nan TOKEN] [--samples N]\n"
            "       [--transform none|'natural logarithm'|'logarithm base 2'|'logarithm base 10']\n"
            "       [--threads N]\n"
-           "       kinc run export-emx --input EMX --output TXT [--nan TOKEN]\n");
+           "       kinc run export-emx --input EMX --output TXT [--nan TOKEN]\n"
+           "       kinc run rmt --input CMX --log LOG [--tstart F] [--tstep F] [--tstop F]\n"
+           "       [--minpace N] [--maxpace N] [--bins N] [--device N]\n");
    return 2;
+}
+
+int run_rmt(int argc, char** argv)
+{
+   kinc_rmt_params p;
+   kinc_rmt_default_params(&p);
+   std::string in, log;
+   int device = 0;
+   for ( int i = 3; i < argc; i += 2 )
+   {
+      if ( i + 1 >= argc ) return usage();
+      const std::string k = argv[i], v = argv[i + 1];
+      if ( k == "--input" ) in = v;
+      else if ( k == "--log" ) log = v;
+      else if ( k == "--tstart" ) p.threshold_start = strtof(v.c_str(), nullptr);
+      else if ( k == "--tstep" ) p.threshold_step = strtof(v.c_str(), nullptr);
+      else if ( k == "--tstop" ) p.threshold_stop = strtof(v.c_str(), nullptr);
+      else if ( k == "--minpace" ) p.min_pace = atoi(v.c_str());
+      else if ( k == "--maxpace" ) p.max_pace = atoi(v.c_str());
+      else if ( k == "--bins" ) p.bins = atoi(v.c_str());
+      else if ( k == "--device" ) device = atoi(v.c_str());
+      else return usage();
+   }
+   if ( in.empty() || log.empty() )
+   {
+      fprintf(stderr, "Did not get valid input or logfile arguments.\n");
+      return 2;
+   }
+   float t = 0;
+   auto t0 = std::chrono::steady_clock::now();
+   if ( kinc_rmt_run_file(&p, in.c_str(), log.c_str(), device, &t) != KINC_OK )
+   {
+      fprintf(stderr, "%s\n", kinc_rmt_last_error(nullptr));
+      return 1;
+   }
+   double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+   fprintf(stderr, "threshold %g in %.2f s\n", (double)t, s);
+   return 0;
 }
 
 /* import-emx / export-emx */
@@ -131,6 +175,7 @@ int main(int argc, char** argv)
    if ( argc < 3 || std::string(argv[1]) != "run" ) return usage();
    if ( std::string(argv[2]) == "import-emx" ) return run_convert(true, argc, argv);
    if ( std::string(argv[2]) == "export-emx" ) return run_convert(false, argc, argv);
+   if ( std::string(argv[2]) == "rmt" ) return run_rmt(argc, argv);
    if ( std::string(argv[2]) != "similarity" ) return usage();
    kinc_sim_params p;
    kinc_sim_default_params(&p);

@@ -54,7 +54,7 @@ def lib():
                                             ctypes.c_int, ctypes.c_int]),
             "oracle_rmt_run": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, f32, f32, f32, f32,
                                               f32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                              ctypes.c_int, P, P, P, ctypes.c_int, P]),
+                                              ctypes.c_int, P, P, P, ctypes.c_int, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -255,8 +255,9 @@ def rmt_run(matrix, start=0.99, step=0.001, stop=0.5, chi1=99.607, chi2=200.0, m
     ls = np.empty(cap, np.int32)
     lc = np.empty(cap, np.float32)
     ft = ctypes.c_float()
+    steps = ctypes.c_int()
     rc = lib().oracle_rmt_run(m.ctypes.data, N, K, start, step, stop, chi1, chi2, min_eigen,
                               min_pace, max_pace, bins, lt.ctypes.data, ls.ctypes.data,
-                              lc.ctypes.data, cap, ctypes.byref(ft))
-    n = min(rc, cap) if rc > 0 else min(cap, int(round((start - stop) / step)) + 1)
+                              lc.ctypes.data, cap, ctypes.byref(ft), ctypes.byref(steps))
+    n = min(steps.value, cap)
     return rc, ft.value, list(zip(lt[:n].tolist(), ls[:n].tolist(), lc[:n].tolist()))

@@ -369,13 +369,15 @@ float oracle_rmt_chi_square(const float* eigens, int n, int min_eigen, int min_p
  * float, as the reference accumulates it) until the largest chi-square
  * after the last sub-critical one reaches chi2.  Per evaluated threshold t
  * the log line is (t, size, chi).  Returns the number of thresholds
- * evaluated (<= cap) and the final threshold, or -1 if the stop threshold
- * was passed first (the reference's "RMT Threshold Error"), -2 on a GSL
- * failure.
+ * evaluated and the final threshold, or -1 if the stop threshold was
+ * passed first (the reference's "RMT Threshold Error"), -2 on a GSL failure
+ * (that step is not logged: the reference aborts inside it); *steps_out =
+ * the thresholds logged.
  */
 int oracle_rmt_run(const float* matrix, int N, int K, float start, float step, float stop,
                    float chi1, float chi2, int min_eigen, int min_pace, int max_pace, int bins,
-                   float* log_t, int32_t* log_size, float* log_chi, int cap, float* final_threshold)
+                   float* log_t, int32_t* log_size, float* log_chi, int cap, float* final_threshold,
+                   int* steps_out)
 {
    float* maxima = (float*)malloc(sizeof(float) * (size_t)N * K);
    int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * (size_t)N * K);
@@ -429,5 +431,6 @@ int oracle_rmt_run(const float* matrix, int N, int K, float start, float step, f
    free(maxima);
    free(idx);
    *final_threshold = finalThreshold;
+   *steps_out = steps;
    return rc ? rc : steps;
 }

@@ -36,8 +36,9 @@ def test_library_exports(name, native):
 
 
 def test_binding_covers_headers():
-    from kinc_amd import formats, serving
-    bound = {n for n, _, _ in _native.SIGNATURES} | set(formats.SIGNATURES) | set(serving.SIGNATURES)
+    from kinc_amd import formats, rmt, serving
+    bound = ({n for n, _, _ in _native.SIGNATURES} | set(formats.SIGNATURES)
+             | set(serving.SIGNATURES) | set(rmt.SIGNATURES))
     missing = set(declared_functions()) - bound
     assert not missing, missing
 
@@ -72,3 +73,13 @@ def test_create_rejects_bad_params_before_device(native, bad, code):
     with pytest.raises(native.KincError) as e:
         native.Engine(np.zeros((4, 8), np.float32), p)
     assert e.value.code == code
+
+
+def test_rmt_default_params_match_reference(native):
+    # src/core/rmt.h:31-40
+    from kinc_amd import rmt
+    p = rmt.make_params()
+    assert abs(p.threshold_start - 0.99) < 1e-7 and abs(p.threshold_step - 0.001) < 1e-9
+    assert p.threshold_stop == 0.5 and abs(p.chi_square1 - 99.607) < 1e-4
+    assert p.chi_square2 == 200 and p.min_eigenvalues == 50
+    assert (p.min_pace, p.max_pace, p.bins) == (10, 40, 60)
