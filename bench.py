@@ -16,7 +16,9 @@ this host (which also re-checks bit-exact parity on that sample).
 
 A serving leg follows: a Poisson stream of gene-set queries against a
 resident 60,000 x 475 matrix (GMM + Pearson, |r| >= 0.5), with p50/p99
-latency beside pairs/s, and the latency of one 100-gene query alone.
+latency beside pairs/s, and the latency of one 100-gene query alone; then the
+RMT threshold sweep on a 5,000-gene correlation matrix (and on a 700-gene one
+beside the CPU oracle).
 
 Usage: python bench.py [--gpus 1] [--steps 8] [--warmup 1] [--no-serve]
 """
@@ -284,6 +286,51 @@ def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
     return out
 
 
+def rmt_leg(gemgen, with_cpu):
+    """The RMT threshold sweep (include/kinc_rmt.h) on a 5,000-gene matrix
+    of 1,000 independent pairs and a 3,000-gene correlated block, and on the
+    700-gene version of it beside the CPU oracle (Jacobi eigensolver)."""
+    from kinc_amd import rmt
+    out = {}
+    m = gemgen.rmt_pairs_and_block(1000, 3000, seed=2)
+    t0 = time.perf_counter()
+    h = rmt.Rmt.from_dense(m)
+    t1 = time.perf_counter()
+    rc, thr, rows = h.run()
+    t2 = time.perf_counter()
+    if rc != 0:
+        log("rmt: %s" % h.last_error())
+    h.close()
+    out.update({
+        "workload": "RMT threshold sweep from 0.99 by 0.001 on 5000 genes (1000 independent "
+                    "pairs r~U(0.55,0.99) + a 3000-gene block r=0.5405+-2e-4), 1 cluster",
+        "records": int(np.count_nonzero(m) // 2), "thresholds": len(rows),
+        "final_threshold": thr, "largest_pruned": max(r[1] for r in rows),
+        "upload_s": t1 - t0, "sweep_s": t2 - t1,
+        "ms_per_threshold": (t2 - t1) / max(1, len(rows)) * 1e3})
+    log("rmt: %d thresholds in %.2fs (final %.4f, largest %d)"
+        % (len(rows), t2 - t1, thr, out["largest_pruned"]))
+    small = gemgen.rmt_pairs_and_block(100, 500, seed=1)
+    h = rmt.Rmt.from_dense(small)
+    t0 = time.perf_counter()
+    rc, thr_s, rows_s = h.run()
+    gpu_s = time.perf_counter() - t0
+    h.close()
+    out["small"] = {"genes": 700, "thresholds": len(rows_s), "sweep_s": gpu_s,
+                    "final_threshold": thr_s}
+    if with_cpu:
+        import oracle
+        t0 = time.process_time()
+        orc, othr, orows = oracle.rmt_run(small)
+        cpu_s = time.process_time() - t0
+        out["small"]["cpu_oracle"] = {"sweep_s": cpu_s, "cores": 1, "kind": "port",
+                                      "final_threshold": othr,
+                                      "same_threshold": othr == thr_s}
+        log("rmt small: gpu %.2fs, cpu oracle %.2fs, thresholds %s / %s"
+            % (gpu_s, cpu_s, thr_s, othr))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +344,7 @@ def main():
     ap.add_argument("--config", default="H", choices=["A", "B", "H", "C"],
                     help="workload (H is the headline; the others are reported for comparison)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-rmt", action="store_true", help="skip the RMT threshold-sweep leg")
     ap.add_argument("--serve-queries", type=int, default=24)
     ap.add_argument("--serve-qps", type=float, default=0.5)
     ap.add_argument("--serve-max-genes", type=int, default=16)
@@ -415,6 +463,10 @@ def main():
                                  "bit-exact on %d/%d" % (mn, mn // used_t, used_t, mn - mbad, mn),
                        "single_thread": {"value": rate, "cores": 1, "sample": cpu["sample"]}}
 
+    rmt_obj = None
+    if not args.no_rmt:
+        rmt_obj = rmt_leg(gemgen, not args.no_cpu)
+
     serving_obj = None
     if not args.no_serve:
         serving_obj = serve_leg(native, gemgen, args.serve_queries, args.serve_qps,
@@ -450,6 +502,7 @@ def main():
         }),
         "cpu_baseline": cpu,
         "serving": serving_obj,
+        "rmt": rmt_obj,
     }
     print(json.dumps(line), flush=True)
 

@@ -121,3 +121,26 @@ def make(name, G=None, S=None):
         return latent(G, S, c["seed"], c.get("nan_rate", 0.05))
     return modules(G, S, c["seed"], nan_rate=c.get("nan_rate", 0.10),
                    nan_beta=c.get("nan_beta", False), degenerate=c.get("degenerate", 0.0))
+
+
+def rmt_pairs_and_block(n_pairs=100, block=500, seed=1, rb=0.5405, eps=2e-4):
+    """A correlation matrix [G][G][1] for the RMT analytic whose pruned
+    versions stay positive definite (so the reference's |value|-ordered
+    eigenvalues are increasing spline knots): n_pairs isolated gene pairs
+    with r ~ U(0.55, 0.99) (eigenvalues 1 - r and 1 + r: independent levels,
+    Poisson spacings) and one block of `block` genes whose correlations lie
+    within eps of rb (it enters between the thresholds 0.541 and 0.540; its
+    levels near 1 - rb repel like a GOE matrix's).  Genes are shuffled."""
+    rng = np.random.default_rng(seed)
+    G = 2 * n_pairs + block
+    m = np.zeros((G, G), np.float32)
+    r = rng.uniform(0.55, 0.99, n_pairs).astype(np.float32)
+    p = np.arange(n_pairs)
+    m[2 * p, 2 * p + 1] = r
+    m[2 * p + 1, 2 * p] = r
+    e = np.triu(rng.uniform(-eps, eps, (block, block)), 1)
+    blk = (rb + e + e.T).astype(np.float32)
+    np.fill_diagonal(blk, 0)
+    m[2 * n_pairs:, 2 * n_pairs:] = blk
+    perm = rng.permutation(G)
+    return np.ascontiguousarray(m[np.ix_(perm, perm)][:, :, None])

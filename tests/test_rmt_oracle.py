@@ -129,26 +129,10 @@ def test_chi_square_knots_must_increase():
     assert oracle.rmt_chi_square(e) == -2
 
 
-def pairs_and_block(n_pairs=100, block=500, seed=1, rb=0.5405, eps=2e-4):
-    """A correlation matrix whose pruned versions stay positive definite, so
-    the reference's |.|-ordered eigenvalues are increasing spline knots:
-    n_pairs isolated gene pairs with r uniform in [0.55, 0.99] (each adds the
-    eigenvalues 1 - r and 1 + r: independent levels, Poisson spacings), and
-    one block whose correlations lie within eps of rb (enters between the
-    thresholds 0.541 and 0.540; its levels near 1 - rb repel like a GOE
-    matrix's)."""
-    rng = np.random.default_rng(seed)
-    N = 2 * n_pairs + block
-    m = np.zeros((N, N), np.float32)
-    r = rng.uniform(0.55, 0.99, n_pairs).astype(np.float32)
-    for p in range(n_pairs):
-        m[2 * p, 2 * p + 1] = m[2 * p + 1, 2 * p] = r[p]
-    e = np.triu(rng.uniform(-eps, eps, (block, block)), 1)
-    blk = (rb + e + e.T).astype(np.float32)
-    np.fill_diagonal(blk, 0)
-    m[2 * n_pairs:, 2 * n_pairs:] = blk
-    perm = rng.permutation(N)
-    return np.ascontiguousarray(m[np.ix_(perm, perm)][:, :, None])
+def pairs_and_block(n_pairs=100, block=500, seed=1):
+    """See kinc_amd.gemgen.rmt_pairs_and_block."""
+    from kinc_amd import gemgen
+    return gemgen.rmt_pairs_and_block(n_pairs, block, seed)
 
 
 def modular_corr(N, modules, seed, noise=1.0):
