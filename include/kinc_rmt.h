@@ -10,11 +10,12 @@
  *   - the rows/columns (gene, cluster) whose largest |r| reaches t are kept
  *     (RMT::computeMaximums/computePruneMatrix, rmt.cpp:160-236);
  *   - the pruned matrix (|r| >= t between genes of the same cluster index,
- *     ones on the diagonal) is block-diagonal by cluster index, so each block
- *     is gathered on the device in double and its eigenvalues are computed
- *     by rocSOLVER (dsyevd); together they are the eigenvalues GSL's symmv
- *     returns for the whole matrix (rmt.cpp:243-275), ordered by increasing
- *     |value| and rounded to float;
+ *     ones on the diagonal) is the direct sum of the connected components of
+ *     that graph; each component that changed since the previous threshold
+ *     is gathered on the device in double and solved by rocSOLVER (dsyevd),
+ *     one- and two-row components in closed form; together their
+ *     eigenvalues are the ones GSL's symmv returns for the whole matrix
+ *     (rmt.cpp:243-275), ordered by increasing |value| and rounded to float;
  *   - the chi-square of their nearest-neighbour spacings against a Poisson
  *     distribution (rmt.cpp:282-439: unique values, Akima unfolding at paces
  *     min_pace..max_pace, a histogram of bins bins on [0, 3)) decides: the

@@ -2,16 +2,17 @@
  * rmt.hip — the RMT threshold analytic on one MI355X (include/kinc_rmt.h).
  *
  * Device side: the correlation matrix as K dense G x G float planes (HBM),
- * row maxima per (gene, cluster), and per threshold a gather of each
- * cluster's pruned block into a double matrix whose eigenvalues rocSOLVER's
- * dsyevd computes.  Host side: the sweep of RMT::process
- * (src/core/rmt.cpp:33-109) and the chi-square of the eigenvalues
- * (rmt.cpp:282-439), a few thousand values per threshold.
+ * and the gather of a pruned block into a double matrix whose eigenvalues
+ * rocSOLVER's dsyevd computes.  Host side: the sweep of RMT::process
+ * (src/core/rmt.cpp:33-109), the connected components of the pruned matrix,
+ * and the chi-square of the eigenvalues (rmt.cpp:282-439).
  *
  * The pruned matrix of the reference (computePruneMatrix, rmt.cpp:196-236)
- * only links rows of the same cluster index, so it is the direct sum of one
- * block per cluster index and its spectrum is the union of theirs; the
- * blocks are solved separately (sum of n_k^3 instead of (sum n_k)^3).
+ * links rows of the same cluster index whose |r| reaches the threshold, with
+ * ones on the diagonal: it is the direct sum of one block per connected
+ * component of that graph, and its spectrum is the union of theirs.  Each
+ * component is solved on its own, and only when it changed since the
+ * previous (higher) threshold.
  */
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
@@ -39,28 +40,6 @@ __global__ void k_rmt_scatter(const int32_t* __restrict__ x, const int32_t* __re
    float* plane = corr + (size_t)k[i] * (size_t)G * (size_t)G;
    plane[(size_t)x[i] * G + y[i]] = r[i];
    plane[(size_t)y[i] * G + x[i]] = r[i];
-}
-
-/* maxima[g*K + k] = max(0, max_j |corr[k][g][j]|) (computeMaximums,
- * rmt.cpp:160-189; the diagonal holds 0).  One 256-thread workgroup per
- * (gene, cluster) row; the row is read once, coalesced. */
-__global__ __launch_bounds__(256) void k_rmt_maxima(const float* __restrict__ corr, int G, int K,
-                                                    float* __restrict__ maxima)
-{
-   const int row = blockIdx.x;           /* k * G + g */
-   const int k = row / G, g = row - k * G;
-   const float* p = corr + ((size_t)k * G + g) * (size_t)G;
-   float m = 0.0f;
-   for ( int j = threadIdx.x; j < G; j += 256 ) m = fmaxf(m, fabsf(p[j]));
-   __shared__ float red[4];
-   for ( int off = 32; off >= 1; off >>= 1 ) m = fmaxf(m, __shfl_xor(m, off));
-   if ( (threadIdx.x & 63) == 0 ) red[threadIdx.x >> 6] = m;
-   __syncthreads();
-   if ( threadIdx.x == 0 )
-   {
-      m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-      maxima[(size_t)g * K + k] = m;
-   }
 }
 
 /* A[i][j] = 1 on the diagonal, else corr[idx_i][idx_j] if |.| >= t, else 0:
@@ -209,14 +188,36 @@ struct kinc_rmt
    hipStream_t stream = nullptr;
    rocblas_handle blas = nullptr;
    float* d_corr = nullptr;
-   float* d_max = nullptr;
    int32_t* d_idx = nullptr;
    double* d_A = nullptr;
    double* d_D = nullptr;
    double* d_E = nullptr;
    rocblas_int* d_info = nullptr;
-   std::vector<float> maxima;
    std::string err;
+
+   /* Incremental state of a sweep.  Nodes are (gene, cluster) rows, id
+    * g*K + k; an edge is a record with |r| >= t.  A node is in the pruned
+    * matrix exactly when one of its edges is (its maximum |r| reaches t,
+    * computeMaximums/computePruneMatrix), and the pruned matrix is the
+    * direct sum of the connected components of this graph, so its spectrum
+    * is the union of theirs.  As t decreases, edges are only added: the
+    * components are kept in a union-find and a component's eigenvalues are
+    * recomputed only when it gained an edge. */
+   struct Edge
+   {
+      int32_t a, b;
+      float r;
+   };
+   std::vector<Edge> edges;                 /* by decreasing |r| */
+   size_t next = 0;
+   float last_t = INFINITY;
+   std::vector<int32_t> parent;             /* -1: not in the pruned matrix */
+   std::vector<std::vector<int32_t>> members;
+   std::vector<std::vector<double>> eig;
+   std::vector<float> pair_r;               /* the edge of a two-node component */
+   std::vector<int64_t> stamp;
+   std::vector<int32_t> dirty;
+   int64_t step = 0;
 };
 
 #define RMT_HIP(h, call)                                                                     \
@@ -249,7 +250,6 @@ extern "C" void kinc_rmt_destroy(kinc_rmt* h)
    (void)hipSetDevice(h->device);
    if ( h->blas ) rocblas_destroy_handle(h->blas);
    (void)hipFree(h->d_corr);
-   (void)hipFree(h->d_max);
    (void)hipFree(h->d_idx);
    (void)hipFree(h->d_A);
    (void)hipFree(h->d_D);
@@ -338,7 +338,6 @@ extern "C" int kinc_rmt_create(const kinc_rmt_params* params, int32_t G, int32_t
    }
    CREATE_HIP(hipMalloc(&h->d_corr, sizeof(float) * plane * (size_t)K));
    CREATE_HIP(hipMemsetAsync(h->d_corr, 0, sizeof(float) * plane * (size_t)K, h->stream));
-   CREATE_HIP(hipMalloc(&h->d_max, sizeof(float) * (size_t)G * K));
    CREATE_HIP(hipMalloc(&h->d_idx, sizeof(int32_t) * (size_t)G));
    CREATE_HIP(hipMalloc(&h->d_A, sizeof(double) * plane));
    CREATE_HIP(hipMalloc(&h->d_D, sizeof(double) * (size_t)G));
@@ -379,48 +378,136 @@ extern "C" int kinc_rmt_create(const kinc_rmt_params* params, int32_t G, int32_t
       (void)hipFree(dr);
       CREATE_HIP(e);
    }
-   k_rmt_maxima<<<(unsigned)(G * K), 256, 0, h->stream>>>(h->d_corr, G, K, h->d_max);
-   CREATE_HIP(hipGetLastError());
-   h->maxima.resize((size_t)G * K);
-   CREATE_HIP(hipMemcpyAsync(h->maxima.data(), h->d_max, sizeof(float) * (size_t)G * K,
-                             hipMemcpyDeviceToHost, h->stream));
    CREATE_HIP(hipStreamSynchronize(h->stream));
 #undef CREATE_HIP
+   h->edges.reserve((size_t)n);
+   for ( int64_t i = 0; i < n; ++i )
+   {
+      if ( corr[i] != 0.0f ) h->edges.push_back({x[i] * K + cluster[i], y[i] * K + cluster[i], corr[i]});
+   }
+   std::stable_sort(h->edges.begin(), h->edges.end(),
+                    [](const kinc_rmt::Edge& a, const kinc_rmt::Edge& b) { return std::fabs(a.r) > std::fabs(b.r); });
+   const size_t nodes = (size_t)G * (size_t)K;
+   h->parent.assign(nodes, -1);
+   h->members.resize(nodes);
+   h->eig.resize(nodes);
+   h->pair_r.assign(nodes, 0.0f);
+   h->stamp.assign(nodes, -1);
    *out = h;
+   return KINC_OK;
+}
+
+static int32_t find_root(kinc_rmt* h, int32_t v)
+{
+   while ( h->parent[v] != v )
+   {
+      h->parent[v] = h->parent[h->parent[v]];
+      v = h->parent[v];
+   }
+   return v;
+}
+
+static void activate(kinc_rmt* h, int32_t v)
+{
+   if ( h->parent[v] >= 0 ) return;
+   h->parent[v] = v;
+   h->members[v].assign(1, v);
+   h->eig[v].clear();
+   h->dirty.push_back(v);
+}
+
+static void reset_sweep(kinc_rmt* h)
+{
+   for ( size_t v = 0; v < h->parent.size(); ++v )
+   {
+      if ( h->parent[v] < 0 ) continue;
+      h->parent[v] = -1;
+      h->members[v].clear();
+      h->eig[v].clear();
+   }
+   h->next = 0;
+   h->dirty.clear();
+}
+
+/* Eigenvalues of one component: closed forms for one node (1) and two nodes
+ * (1 - r, 1 + r, exact in double), else its block gathered on the device
+ * and solved by rocSOLVER. */
+static int solve_component(kinc_rmt* h, int32_t root, float t)
+{
+   std::vector<double>& w = h->eig[root];
+   const std::vector<int32_t>& mem = h->members[root];
+   const int c = (int)mem.size();
+   if ( c == 1 )
+   {
+      w.assign(1, 1.0);
+      return KINC_OK;
+   }
+   if ( c == 2 )
+   {
+      const double r = std::fabs((double)h->pair_r[root]);
+      w = {1.0 - r, 1.0 + r};
+      return KINC_OK;
+   }
+   const int G = h->G, K = h->K;
+   const int k = mem[0] % K;
+   std::vector<int32_t> genes((size_t)c);
+   for ( int i = 0; i < c; ++i ) genes[i] = mem[i] / K;
+   std::sort(genes.begin(), genes.end());
+   RMT_HIP(h, hipMemcpyAsync(h->d_idx, genes.data(), sizeof(int32_t) * c, hipMemcpyHostToDevice, h->stream));
+   dim3 grid((unsigned)c, (unsigned)((c + 255) / 256));
+   k_rmt_gather<<<grid, 256, 0, h->stream>>>(h->d_corr + (size_t)k * G * G, G, h->d_idx, c, t, h->d_A);
+   RMT_HIP(h, hipGetLastError());
+   rocblas_status st = rocsolver_dsyevd(h->blas, rocblas_evect_none, rocblas_fill_lower, c, h->d_A, c,
+                                        h->d_D, h->d_E, h->d_info);
+   if ( st != rocblas_status_success )
+      return set_err(&h->err, KINC_E_DEVICE, "rocsolver_dsyevd failed: " + std::to_string((int)st));
+   w.resize((size_t)c);
+   rocblas_int info = 0;
+   RMT_HIP(h, hipMemcpyAsync(w.data(), h->d_D, sizeof(double) * c, hipMemcpyDeviceToHost, h->stream));
+   RMT_HIP(h, hipMemcpyAsync(&info, h->d_info, sizeof(info), hipMemcpyDeviceToHost, h->stream));
+   RMT_HIP(h, hipStreamSynchronize(h->stream));
+   if ( info != 0 )
+      return set_err(&h->err, KINC_E_DEVICE, "dsyevd did not converge (info " + std::to_string(info) + ")");
    return KINC_OK;
 }
 
 /* Eigenvalues of the matrix pruned at t, |.|-ascending (double). */
 static int eigenvalues(kinc_rmt* h, float t, std::vector<double>& w)
 {
-   w.clear();
-   const int G = h->G, K = h->K;
-   std::vector<int32_t> idx;
-   for ( int k = 0; k < K; ++k )
+   if ( t > h->last_t ) reset_sweep(h);
+   h->last_t = t;
+   ++h->step;
+   /* add the edges that reach t; mark the components they touch */
+   while ( h->next < h->edges.size() && std::fabs(h->edges[h->next].r) >= t )
    {
-      idx.clear();
-      for ( int g = 0; g < G; ++g )
+      const kinc_rmt::Edge& e = h->edges[h->next++];
+      activate(h, e.a);
+      activate(h, e.b);
+      int32_t ra = find_root(h, e.a), rb = find_root(h, e.b);
+      if ( ra != rb )
       {
-         if ( h->maxima[(size_t)g * K + k] >= t ) idx.push_back(g);
+         if ( h->members[ra].size() < h->members[rb].size() ) std::swap(ra, rb);
+         h->members[ra].insert(h->members[ra].end(), h->members[rb].begin(), h->members[rb].end());
+         std::vector<int32_t>().swap(h->members[rb]);
+         std::vector<double>().swap(h->eig[rb]);
+         h->parent[rb] = ra;
       }
-      const int nk = (int)idx.size();
-      if ( nk == 0 ) continue;
-      RMT_HIP(h, hipMemcpyAsync(h->d_idx, idx.data(), sizeof(int32_t) * nk, hipMemcpyHostToDevice, h->stream));
-      dim3 grid((unsigned)nk, (unsigned)((nk + 255) / 256));
-      k_rmt_gather<<<grid, 256, 0, h->stream>>>(h->d_corr + (size_t)k * G * G, G, h->d_idx, nk, t, h->d_A);
-      RMT_HIP(h, hipGetLastError());
-      rocblas_status st = rocsolver_dsyevd(h->blas, rocblas_evect_none, rocblas_fill_lower, nk, h->d_A, nk,
-                                           h->d_D, h->d_E, h->d_info);
-      if ( st != rocblas_status_success )
-         return set_err(&h->err, KINC_E_DEVICE, "rocsolver_dsyevd failed: " + std::to_string((int)st));
-      const size_t base = w.size();
-      w.resize(base + (size_t)nk);
-      rocblas_int info = 0;
-      RMT_HIP(h, hipMemcpyAsync(w.data() + base, h->d_D, sizeof(double) * nk, hipMemcpyDeviceToHost, h->stream));
-      RMT_HIP(h, hipMemcpyAsync(&info, h->d_info, sizeof(info), hipMemcpyDeviceToHost, h->stream));
-      RMT_HIP(h, hipStreamSynchronize(h->stream));
-      if ( info != 0 )
-         return set_err(&h->err, KINC_E_DEVICE, "dsyevd did not converge (info " + std::to_string(info) + ")");
+      if ( h->members[ra].size() == 2 ) h->pair_r[ra] = e.r;
+      h->dirty.push_back(ra);
+   }
+   for ( int32_t v : h->dirty )
+   {
+      const int32_t r = find_root(h, v);
+      if ( h->stamp[r] == h->step ) continue;
+      h->stamp[r] = h->step;
+      int rc = solve_component(h, r, t);
+      if ( rc ) return rc;
+   }
+   h->dirty.clear();
+   w.clear();
+   for ( size_t v = 0; v < h->parent.size(); ++v )
+   {
+      if ( h->parent[v] == (int32_t)v ) w.insert(w.end(), h->eig[v].begin(), h->eig[v].end());
    }
    /* gsl_eigen_symmv_sort(GSL_EIGEN_SORT_ABS_ASC) */
    std::stable_sort(w.begin(), w.end(), [](double a, double b) { return std::fabs(a) < std::fabs(b); });
