@@ -112,6 +112,33 @@ int kinc_similarity_writer_process(kinc_similarity_writer* w, int64_t start, int
 /* Rewrite the headers with the final counts and close both files. */
 int kinc_similarity_writer_close(kinc_similarity_writer* w);
 
+/*
+ * ExportCorrelationMatrix::process (src/core/exportcorrelationmatrix.cpp:17-117):
+ * one text line per CMX cluster, in file order:
+ *   x, y, cluster, cluster count, and for pairs with more than one cluster
+ *   the counts of CCM nibbles 1 (in the cluster), 9 (missing), 8 (post-
+ *   clustering outlier), 7 (pre-clustering outlier), 6 (below the minimum
+ *   expression) (else zeros), the correlation ("%g"), and the CCM mask as
+ *   one character '0' + nibble per sample (all '0' for a one-cluster pair);
+ * fields separated by tabs.
+ */
+int kinc_export_cmx_text(const char* ccm_path, const char* cmx_path, const char* out_path);
+
+/*
+ * Extract::process (src/core/extract.cpp:14-275): the network of the
+ * clusters with min_corr <= |r| <= max_corr (a cluster outside is skipped),
+ * as a tab-separated edge list with the header
+ *   Source Target sc Interaction Cluster Num_Clusters Cluster_Samples
+ *   Missing_Samples Cluster_Outliers Pair_Outliers Too_Low Samples
+ * (gene names from the CMX metadata, interaction "co", the counts and mask
+ * as kinc_export_cmx_text, except that a one-cluster pair's mask comes from
+ * the expression matrix: '9' where either gene is NaN, else '1'), and, when
+ * graphml_path is not NULL, the same edges as GraphML with every gene a node.
+ */
+int kinc_extract_network(const char* emx_path, const char* ccm_path, const char* cmx_path,
+                         const char* out_path, const char* graphml_path, float min_corr,
+                         float max_corr);
+
 /* Message of the last failed format call of this thread. */
 const char* kinc_formats_last_error(void);
 

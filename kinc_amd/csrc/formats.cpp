@@ -1129,3 +1129,342 @@ extern "C" int kinc_emx_export_text(const char* emx_path, const char* text_path,
    ok = (fclose(f) == 0) && ok;
    return ok ? KINC_OK : ferr(KINC_E_INVALID, std::string(text_path) + ": write failed");
 }
+
+/* ------------------------------------------- CMX text export and extract */
+
+namespace {
+
+/* A pairwise file (CCM or CMX) read whole: header fields and records. */
+struct PairwiseIn
+{
+   int32_t G = 0, max_clusters = 0, data_size = 0;
+   int64_t pairs = 0, clusters = 0;
+   std::vector<uint8_t> sub;
+   std::vector<uint8_t> payload;
+   std::string meta;
+   size_t base = 0;   /* offset of the first record in payload */
+
+   int32_t x(int64_t i) const { return (int32_t)get32(rec(i)); }
+   int32_t y(int64_t i) const { return (int32_t)get32(rec(i) + 4); }
+   int cluster(int64_t i) const { return (int8_t)rec(i)[8]; }
+   const uint8_t* data(int64_t i) const { return rec(i) + 9; }
+   const uint8_t* rec(int64_t i) const { return payload.data() + base + (size_t)i * (9 + (size_t)data_size); }
+   int64_t pair_index(int64_t i) const { return (int64_t)x(i) * (x(i) - 1) / 2 + y(i); }
+};
+
+int read_pairwise(const char* path, uint32_t type, PairwiseIn& p)
+{
+   FILE* f = fopen(path, "rb");
+   if ( !f ) return ferr(KINC_E_INVALID, std::string(path) + ": cannot open");
+   EnvelopeInfo e;
+   int rc = read_envelope(f, e, path);
+   if ( rc == KINC_OK && e.type != type )
+      rc = ferr(KINC_E_INVALID, std::string(path) + (type == KINC_FILE_CCM ? ": not a CCM file" : ": not a CMX file"));
+   if ( rc == KINC_OK )
+   {
+      p.payload.resize((size_t)e.payload_size);
+      p.meta.resize((size_t)e.meta_size);
+      if ( fseek(f, (long)e.payload_off, SEEK_SET) != 0
+           || fread(p.payload.data(), 1, p.payload.size(), f) != p.payload.size()
+           || fseek(f, (long)e.meta_off, SEEK_SET) != 0
+           || (p.meta.size() && fread(&p.meta[0], 1, p.meta.size(), f) != p.meta.size()) )
+      {
+         rc = ferr(KINC_E_INVALID, std::string(path) + ": short file");
+      }
+   }
+   fclose(f);
+   if ( rc ) return rc;
+   const uint8_t* h = p.payload.data();
+   if ( p.payload.size() < 30 ) return ferr(KINC_E_INVALID, std::string(path) + ": short header");
+   p.G = (int32_t)get32(h);
+   p.max_clusters = (int32_t)get32(h + 4);
+   p.data_size = (int32_t)get32(h + 8);
+   p.pairs = (int64_t)get64(h + 12);
+   p.clusters = (int64_t)get64(h + 20);
+   const size_t off = (size_t)h[28] | ((size_t)h[29] << 8);
+   p.base = 30 + off;
+   if ( p.base > p.payload.size() || p.data_size < 0
+        || p.base + (size_t)p.clusters * (9 + (size_t)p.data_size) > p.payload.size() )
+   {
+      return ferr(KINC_E_INVALID, std::string(path) + ": truncated records");
+   }
+   p.sub.assign(h + 30, h + p.base);
+   return KINC_OK;
+}
+
+/* CCM mask nibble of sample i (CCMatrix::Pair, ccmatrix.cpp:223-243) */
+inline int ccm_nibble(const uint8_t* data, int i)
+{
+   const uint8_t b = data[i / 2];
+   return (i & 1) ? (b >> 4) : (b & 0x0f);
+}
+
+struct MaskStats
+{
+   int samples = 0, missing = 0, post = 0, pre = 0, threshold = 0;
+};
+
+/* the summary counts and the mask text of one CCM cluster */
+void ccm_cluster(const uint8_t* data, int S, MaskStats& st, std::string& mask)
+{
+   st = MaskStats();
+   mask.resize((size_t)S);
+   for ( int i = 0; i < S; ++i )
+   {
+      const int v = ccm_nibble(data, i);
+      switch ( v )
+      {
+      case 1: ++st.samples; break;
+      case 6: ++st.threshold; break;
+      case 7: ++st.pre; break;
+      case 8: ++st.post; break;
+      case 9: ++st.missing; break;
+      default: break;
+      }
+      mask[(size_t)i] = (char)('0' + v);
+   }
+}
+
+/* Walks the CMX pairs in file order with the CCM record of each of their
+ * clusters (both files hold the pairs in increasing index order). */
+struct PairWalker
+{
+   const PairwiseIn& cmx;
+   const PairwiseIn& ccm;
+   int64_t i = 0;      /* first CMX record of the current pair */
+   int64_t n = 0;      /* its cluster count */
+   int64_t j = 0;      /* CCM cursor */
+   int64_t ccm_first = -1;
+
+   PairWalker(const PairwiseIn& a, const PairwiseIn& b) : cmx(a), ccm(b) {}
+
+   bool next()
+   {
+      i += n;
+      if ( i >= cmx.clusters ) return false;
+      const int64_t idx = cmx.pair_index(i);
+      n = 1;
+      while ( i + n < cmx.clusters && cmx.pair_index(i + n) == idx ) ++n;
+      ccm_first = -1;
+      if ( n > 1 )
+      {
+         while ( j < ccm.clusters && ccm.pair_index(j) < idx ) ++j;
+         if ( j < ccm.clusters && ccm.pair_index(j) == idx ) ccm_first = j;
+      }
+      return true;
+   }
+
+   /* the CCM data of cluster k of the current pair, or null */
+   const uint8_t* ccm_data(int k) const
+   {
+      if ( ccm_first < 0 || ccm_first + k >= ccm.clusters ) return nullptr;
+      if ( ccm.pair_index(ccm_first + k) != cmx.pair_index(i) || ccm.cluster(ccm_first + k) != k ) return nullptr;
+      return ccm.data(ccm_first + k);
+   }
+};
+
+float cmx_corr(const PairwiseIn& cmx, int64_t r)
+{
+   float c;
+   memcpy(&c, cmx.data(r), 4);
+   return c;
+}
+
+/* buffered text output */
+struct TextOut
+{
+   FILE* f = nullptr;
+   std::string buf;
+   bool ok = true;
+   int open(const char* path)
+   {
+      f = fopen(path, "wb");
+      if ( !f ) return ferr(KINC_E_INVALID, std::string(path) + ": cannot open for writing");
+      buf.reserve(1 << 20);
+      return KINC_OK;
+   }
+   void put(const std::string& s)
+   {
+      buf += s;
+      if ( buf.size() >= (1u << 20) ) flush();
+   }
+   void flush()
+   {
+      if ( ok && !buf.empty() ) ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+      buf.clear();
+   }
+   bool close()
+   {
+      flush();
+      const bool c = f && fclose(f) == 0;
+      f = nullptr;
+      return ok && c;
+   }
+};
+
+std::string fmt_g(float v)
+{
+   char b[32];
+   snprintf(b, sizeof(b), "%g", (double)v);
+   return b;
+}
+
+int ccm_samples(const PairwiseIn& ccm)
+{
+   return ccm.sub.size() >= 4 ? (int)get32(ccm.sub.data()) : 0;
+}
+
+} // namespace
+
+extern "C" int kinc_export_cmx_text(const char* ccm_path, const char* cmx_path, const char* out_path)
+{
+   if ( !ccm_path || !cmx_path || !out_path )
+      return ferr(KINC_E_INVALID, "Did not get valid input and/or output arguments.");
+   PairwiseIn ccm, cmx;
+   int rc = read_pairwise(ccm_path, KINC_FILE_CCM, ccm);
+   if ( rc == KINC_OK ) rc = read_pairwise(cmx_path, KINC_FILE_CMX, cmx);
+   if ( rc ) return rc;
+   const int S = ccm_samples(ccm);
+   TextOut out;
+   rc = out.open(out_path);
+   if ( rc ) return rc;
+   PairWalker w(cmx, ccm);
+   std::string mask, line;
+   MaskStats st;
+   while ( w.next() )
+   {
+      for ( int k = 0; k < (int)w.n; ++k )
+      {
+         const float corr = cmx_corr(cmx, w.i + k);
+         st = MaskStats();
+         if ( w.n > 1 )
+         {
+            const uint8_t* d = w.ccm_data(k);
+            if ( !d )
+            {
+               out.close();
+               return ferr(KINC_E_INVALID, "CCM has no record for pair (" + std::to_string(cmx.x(w.i)) + ", "
+                                              + std::to_string(cmx.y(w.i)) + ") cluster " + std::to_string(k));
+            }
+            ccm_cluster(d, S, st, mask);
+         }
+         else
+         {
+            mask.assign((size_t)S, '0');
+         }
+         line = std::to_string(cmx.x(w.i)) + "\t" + std::to_string(cmx.y(w.i)) + "\t" + std::to_string(k) + "\t"
+                + std::to_string(w.n) + "\t" + std::to_string(st.samples) + "\t" + std::to_string(st.missing)
+                + "\t" + std::to_string(st.post) + "\t" + std::to_string(st.pre) + "\t"
+                + std::to_string(st.threshold) + "\t" + fmt_g(corr) + "\t" + mask + "\n";
+         out.put(line);
+      }
+   }
+   return out.close() ? KINC_OK : ferr(KINC_E_INVALID, std::string(out_path) + ": write failed");
+}
+
+extern "C" int kinc_extract_network(const char* emx_path, const char* ccm_path, const char* cmx_path,
+                                    const char* out_path, const char* graphml_path, float min_corr,
+                                    float max_corr)
+{
+   if ( !emx_path || !ccm_path || !cmx_path || !out_path )
+      return ferr(KINC_E_INVALID, "Did not get valid input and/or output arguments.");
+   PairwiseIn ccm, cmx;
+   int rc = read_pairwise(ccm_path, KINC_FILE_CCM, ccm);
+   if ( rc == KINC_OK ) rc = read_pairwise(cmx_path, KINC_FILE_CMX, cmx);
+   if ( rc ) return rc;
+   int32_t G = 0, SE = 0;
+   rc = kinc_emx_info(emx_path, &G, &SE);
+   if ( rc ) return rc;
+   std::vector<float> emx((size_t)G * (size_t)SE);
+   rc = kinc_emx_read(emx_path, emx.data());
+   if ( rc ) return rc;
+   const int S = ccm_samples(ccm);
+   if ( SE != S ) return ferr(KINC_E_INVALID, "the expression matrix and the CCM have different sample counts");
+   std::vector<std::string> names;
+   if ( !json_top_value(cmx.meta, "genes", names) || (int)names.size() < cmx.G )
+   {
+      names.clear();
+      for ( int g = 0; g < cmx.G; ++g ) names.push_back(std::to_string(g));
+   }
+   if ( cmx.G > G ) return ferr(KINC_E_INVALID, "the CMX has more genes than the expression matrix");
+
+   /* the mask of one cluster: CCM data for a multi-cluster pair, else the
+    * NaN pattern of the two genes */
+   auto cluster_mask = [&](const PairWalker& w, int k, MaskStats& st, std::string& mask) -> bool {
+      st = MaskStats();
+      if ( w.n > 1 )
+      {
+         const uint8_t* d = w.ccm_data(k);
+         if ( !d ) return false;
+         ccm_cluster(d, S, st, mask);
+         return true;
+      }
+      const float* a = emx.data() + (size_t)cmx.x(w.i) * S;
+      const float* b = emx.data() + (size_t)cmx.y(w.i) * S;
+      mask.resize((size_t)S);
+      for ( int i = 0; i < S; ++i ) mask[(size_t)i] = (std::isnan(a[i]) || std::isnan(b[i])) ? '9' : '1';
+      return true;
+   };
+   auto excluded = [&](float c) { return std::fabs(c) < min_corr || max_corr < std::fabs(c); };
+
+   TextOut out;
+   rc = out.open(out_path);
+   if ( rc ) return rc;
+   out.put("Source\tTarget\tsc\tInteraction\tCluster\tNum_Clusters\tCluster_Samples\tMissing_Samples\t"
+           "Cluster_Outliers\tPair_Outliers\tToo_Low\tSamples\n");
+   std::string mask, line;
+   MaskStats st;
+   {
+      PairWalker w(cmx, ccm);
+      while ( w.next() )
+      {
+         for ( int k = 0; k < (int)w.n; ++k )
+         {
+            const float corr = cmx_corr(cmx, w.i + k);
+            if ( excluded(corr) ) continue;
+            if ( !cluster_mask(w, k, st, mask) )
+            {
+               out.close();
+               return ferr(KINC_E_INVALID, "CCM has no record for a multi-cluster pair");
+            }
+            line = names[(size_t)cmx.x(w.i)] + "\t" + names[(size_t)cmx.y(w.i)] + "\t" + fmt_g(corr) + "\tco\t"
+                   + std::to_string(k) + "\t" + std::to_string(w.n) + "\t" + std::to_string(st.samples) + "\t"
+                   + std::to_string(st.missing) + "\t" + std::to_string(st.post) + "\t" + std::to_string(st.pre)
+                   + "\t" + std::to_string(st.threshold) + "\t" + mask + "\n";
+            out.put(line);
+         }
+      }
+   }
+   if ( !out.close() ) return ferr(KINC_E_INVALID, std::string(out_path) + ": write failed");
+   if ( !graphml_path ) return KINC_OK;
+
+   TextOut gml;
+   rc = gml.open(graphml_path);
+   if ( rc ) return rc;
+   gml.put("<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n"
+           "<graphml xmlns=\"http://graphml.graphdrawing.org/xmlns\"\n"
+           "    xmlns:xsi=\"http://www.w3.org/2001/XMLSchema-instance\"\n"
+           "    xsi:schemaLocation=\"http://graphml.graphdrawing.org/xmlns/1.0/graphml.xsd\">\n"
+           "  <graph id=\"G\" edgedefault=\"undirected\">\n");
+   for ( int g = 0; g < cmx.G; ++g ) gml.put("    <node id=\"" + names[(size_t)g] + "\"/>\n");
+   {
+      PairWalker w(cmx, ccm);
+      while ( w.next() )
+      {
+         for ( int k = 0; k < (int)w.n; ++k )
+         {
+            const float corr = cmx_corr(cmx, w.i + k);
+            if ( excluded(corr) ) continue;
+            if ( !cluster_mask(w, k, st, mask) )
+            {
+               gml.close();
+               return ferr(KINC_E_INVALID, "CCM has no record for a multi-cluster pair");
+            }
+            gml.put("    <edge source=\"" + names[(size_t)cmx.x(w.i)] + "\" target=\"" + names[(size_t)cmx.y(w.i)]
+                    + "\" samples=\"" + mask + "\"/>\n");
+         }
+      }
+   }
+   gml.put("  </graph>\n</graphml>\n");
+   return gml.close() ? KINC_OK : ferr(KINC_E_INVALID, std::string(graphml_path) + ": write failed");
+}

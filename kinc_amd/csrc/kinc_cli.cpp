@@ -15,7 +15,12 @@
  *   kinc run import-emx --input A.txt --output A.emx [--nan TOKEN]
  *        [--samples N] [--transform none|"natural logarithm"|...] [--threads N]
  *   kinc run export-emx --input A.emx --output A.txt [--nan TOKEN]
- * and the RMT threshold analytic after it (rmt_input.cpp:58-136):
+ * the correlation-matrix exporters (exportcorrelationmatrix_input.cpp,
+ * extract_input.cpp):
+ *   kinc run export-cmx --ccm A.ccm --cmx A.cmx --output A.txt
+ *   kinc run extract --emx A.emx --ccm A.ccm --cmx A.cmx --output net.txt
+ *        [--graphml net.graphml] [--mincorr 0.85] [--maxcorr 1]
+ * and the RMT threshold analytic (rmt_input.cpp:58-136):
  *   kinc run rmt --input A.cmx --log A.log [--tstart F] [--tstep F]
  *        [--tstop F] [--minpace N] [--maxpace N] [--bins N] [--device N]
  *
@@ -56,9 +61,46 @@ int usage()
            "       [--transform none|'natural logarithm'|'logarithm base 2'|'logarithm base 10']\n"
            "       [--threads N]\n"
            "       kinc run export-emx --input EMX --output TXT [--nan TOKEN]\n"
+           "       kinc run export-cmx --ccm CCM --cmx CMX --output TXT\n"
+           "       kinc run extract --emx EMX --ccm CCM --cmx CMX --output TXT [--graphml GML]\n"
+           "       [--mincorr F] [--maxcorr F]\n"
            "       kinc run rmt --input CMX --log LOG [--tstart F] [--tstep F] [--tstop F]\n"
            "       [--minpace N] [--maxpace N] [--bins N] [--device N]\n");
    return 2;
+}
+
+/* export-cmx / extract */
+int run_export(bool extract, int argc, char** argv)
+{
+   std::string emx, ccm, cmx, out, gml;
+   float mincorr = 0.85f, maxcorr = 1.0f;
+   for ( int i = 3; i < argc; i += 2 )
+   {
+      if ( i + 1 >= argc ) return usage();
+      const std::string k = argv[i], v = argv[i + 1];
+      if ( k == "--ccm" ) ccm = v;
+      else if ( k == "--cmx" ) cmx = v;
+      else if ( k == "--output" ) out = v;
+      else if ( extract && k == "--emx" ) emx = v;
+      else if ( extract && k == "--graphml" ) gml = v;
+      else if ( extract && k == "--mincorr" ) mincorr = strtof(v.c_str(), nullptr);
+      else if ( extract && k == "--maxcorr" ) maxcorr = strtof(v.c_str(), nullptr);
+      else return usage();
+   }
+   if ( ccm.empty() || cmx.empty() || out.empty() || (extract && emx.empty()) )
+   {
+      fprintf(stderr, "Did not get valid input and/or output arguments.\n");
+      return 2;
+   }
+   const int rc = extract ? kinc_extract_network(emx.c_str(), ccm.c_str(), cmx.c_str(), out.c_str(),
+                                                 gml.empty() ? nullptr : gml.c_str(), mincorr, maxcorr)
+                          : kinc_export_cmx_text(ccm.c_str(), cmx.c_str(), out.c_str());
+   if ( rc != KINC_OK )
+   {
+      fprintf(stderr, "%s\n", kinc_formats_last_error());
+      return 1;
+   }
+   return 0;
 }
 
 int run_rmt(int argc, char** argv)
@@ -176,6 +218,8 @@ int main(int argc, char** argv)
    if ( std::string(argv[2]) == "import-emx" ) return run_convert(true, argc, argv);
    if ( std::string(argv[2]) == "export-emx" ) return run_convert(false, argc, argv);
    if ( std::string(argv[2]) == "rmt" ) return run_rmt(argc, argv);
+   if ( std::string(argv[2]) == "export-cmx" ) return run_export(false, argc, argv);
+   if ( std::string(argv[2]) == "extract" ) return run_export(true, argc, argv);
    if ( std::string(argv[2]) != "similarity" ) return usage();
    kinc_sim_params p;
    kinc_sim_default_params(&p);

@@ -36,6 +36,10 @@ SIGNATURES = {
                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             _P, _P]),
     "kinc_emx_export_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "kinc_export_cmx_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "kinc_extract_network": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                            ctypes.c_char_p, ctypes.c_char_p, ctypes.c_float,
+                                            ctypes.c_float]),
 }
 # ExpressionMatrix::TRANSFORM_NAMES (src/core/expressionmatrix.cpp:8-14)
 TRANSFORMS = ["none", "natural logarithm", "logarithm base 2", "logarithm base 10"]
@@ -85,6 +89,18 @@ def export_text(emx_path, text_path, nan=""):
     """ExportExpressionMatrix: an EMX file to text (kinc_emx_export_text)."""
     _check(_bind().kinc_emx_export_text(emx_path.encode(), text_path.encode(),
                                         (nan or "").encode()))
+
+
+def export_cmx_text(ccm, cmx, out):
+    """ExportCorrelationMatrix: CCM + CMX to text (kinc_export_cmx_text)."""
+    _check(_bind().kinc_export_cmx_text(ccm.encode(), cmx.encode(), out.encode()))
+
+
+def extract_network(emx, ccm, cmx, out, graphml=None, min_corr=0.85, max_corr=1.0):
+    """Extract: the thresholded network as an edge list (+ GraphML)."""
+    _check(_bind().kinc_extract_network(emx.encode(), ccm.encode(), cmx.encode(), out.encode(),
+                                        graphml.encode() if graphml else None, min_corr,
+                                        max_corr))
 
 
 def read_emx_native(path):
