@@ -315,7 +315,7 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    a.blockN = h->d_blockN;
    a.chunkMax = h->d_chunkMax;
    a.np2 = next_pow2(S);
-   a.tile_stride = 64 + 4;
+   a.tile_stride = KINC_TILE_STRIDE;
    a.mem_slots = (S + 63) / 64;
    const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
    const size_t sort_bytes = sizeof(float) * 2 * (size_t)a.np2;

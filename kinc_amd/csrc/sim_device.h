@@ -18,6 +18,11 @@
 #define KINC_MAX_REG_SAMPLES (64 * KINC_MAX_P)
 /* Largest sample count (Spearman pads to the next power of two, 32768). */
 #define KINC_MAX_SAMPLES 32768
+/* Floats per chain row of the LDS term tile: 64 points and 4 of padding, so
+ * the 16-byte reads of one row per lane spread over the banks.  A compile-
+ * time constant lets every tile address fold into the LDS instruction's
+ * offset field. */
+#define KINC_TILE_STRIDE 68
 
 /* Pair-kernel argument block (passed by value). */
 struct SimArgs
@@ -73,7 +78,7 @@ struct SimArgs
    int32_t scr_y;            /* int8[2][mem_slots][64] K-means labels */
    int32_t mem_slots;        /* ceil(S / 64) */
    int32_t np2;              /* next power of two >= S (min 2) */
-   int32_t tile_stride;      /* floats per chain row of the term tile */
+   int32_t tile_stride;      /* KINC_TILE_STRIDE */
 };
 
 #endif

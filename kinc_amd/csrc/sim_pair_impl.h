@@ -580,7 +580,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
                                           bool want_entropy)
 {
    const int lane = lane_id();
-   const int STR = a.tile_stride;
+   constexpr int STR = KINC_TILE_STRIDE;
    const int nslots = (N + WAVE - 1) / WAVE;
    float* tile = L.tile;
    float* prm = L.prm;
@@ -851,7 +851,7 @@ template <class ST, int K>
 __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int N, ST& st)
 {
    const int lane = lane_id();
-   const int STR = a.tile_stride;
+   constexpr int STR = KINC_TILE_STRIDE;
    const int nslots = (N + WAVE - 1) / WAVE;
    float* tile = L.tile;
    float* means = L.prm;
