@@ -186,31 +186,38 @@ def host_threads():
 
 
 def cpu_baseline_threads(oracle, native, x, flags, blocks, seconds, gpu_out, rate1, threads):
-    """The oracle on every host thread at once, one work-block prefix per
-    thread (the reference parallelises by blocks too), sized from the
-    single-thread rate so the leg takes about `seconds` of wall time.  The
-    prefixes are those of the timed blocks, so the device outputs check them
-    bit for bit.  Returns pairs/s (wall), pairs, wall seconds, mismatches."""
+    """The oracle on every host thread at once (the reference parallelises by
+    work blocks), sized from the single-thread rate so the leg takes about
+    `seconds` of wall time.  Each thread takes a run of pairs from the start
+    of a timed block; without Pearson (whose stale workspace ties a pair to
+    its block) a block may give several consecutive runs, one per thread, so
+    every host thread has work.  The device outputs check every pair bit for
+    bit.  Returns pairs/s (wall), pairs, wall seconds, mismatches, threads."""
     from concurrent.futures import ThreadPoolExecutor
     from parity_util import count_mismatching_pairs as compare_block
     p = native.make_params(**flags)
-    tasks = [(i, b) for i, b in enumerate(blocks)][:threads]
-    per_block = int(max(8, min(BLOCK, rate1 * seconds)))
+    per_run = int(max(8, min(BLOCK, rate1 * seconds)))
+    runs_per_block = 1
+    if flags.get("corrmethod", "pearson") != "pearson":
+        runs_per_block = max(1, -(-threads // len(blocks)))
+        per_run = min(per_run, BLOCK // runs_per_block)
+    tasks = [(i, b, r * per_run) for i, b in enumerate(blocks) for r in range(runs_per_block)]
+    tasks = tasks[:threads]
 
-    def work(ib):
-        i, b = ib
-        return i, oracle.execute_block(p, x, b * BLOCK, per_block)
+    def work(t):
+        i, b, off = t
+        return i, off, oracle.execute_block(p, x, b * BLOCK + off, per_run)
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=len(tasks)) as ex:
         results = list(ex.map(work, tasks))
     wall = time.perf_counter() - t0
     mismatches = 0
-    for i, want in results:
-        got = tuple(a[i][:per_block] for a in
+    for i, off, want in results:
+        got = tuple(a[i][off:off + per_run] for a in
                     (gpu_out["K"], gpu_out["labels"], gpu_out["corr"], gpu_out["iters"]))
         mismatches += compare_block(got, want)
-    done = per_block * len(tasks)
+    done = per_run * len(tasks)
     return done / wall, done, wall, mismatches, len(tasks)
 
 
@@ -451,14 +458,14 @@ def main():
                          "C oracle (-O2, no FP contraction); GPU output bit-exact on %d/%d"
                          % (n, len(blocks), checked - bad, checked)}
         if not args.no_cpu_threads:
-            threads = min(host_threads(), len(blocks))
+            threads = host_threads()
             if threads > 1:
                 mrate, mn, mwall, mbad, used_t = cpu_baseline_threads(
                     oracle, native, x, flags, blocks, args.cpu_seconds * 2 / 3, out, rate, threads)
                 log("cpu oracle, %d threads: %d pairs in %.1fs wall -> %.2f pairs/s; parity %d/%d "
                     "mismatching" % (used_t, mn, mwall, mrate, mbad, mn))
                 cpu = {"value": mrate, "unit": "pairs/s", "cores": used_t, "kind": "port",
-                       "sample": "%d pairs: one %d-pair prefix of a timed work block per thread, "
+                       "sample": "%d pairs: one run of %d pairs of a timed work block per thread, "
                                  "C oracle (-O2, no FP contraction) on %d host threads; GPU output "
                                  "bit-exact on %d/%d" % (mn, mn // used_t, used_t, mn - mbad, mn),
                        "single_thread": {"value": rate, "cores": 1, "sample": cpu["sample"]}}
