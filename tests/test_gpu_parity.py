@@ -41,6 +41,27 @@ CASES = [
     ("gmm_spearman_E_S5000", dict(kind="modules", G=10, S=5000, seed=6, nan_rate=0.10,
                                   degenerate=0.2),
      dict(clusmethod="gmm", corrmethod="spearman", crit="BIC"), 0, 45),
+    # parameter corners and slot-count boundaries: a sample count at the edge
+    # of a register slot (64, 65), of the register range (1024) and just past
+    # it (1025, the memory store), one cluster only, minimum samples above
+    # most pairs' N, ICL with K-means, minclus > 1
+    ("kmeans_spearman_icl_out", dict(kind="modules", G=24, S=150, seed=30, nan_rate=0.10),
+     dict(clusmethod="kmeans", corrmethod="spearman", crit="ICL", preout=True, postout=True),
+     0, 120),
+    ("gmm_pearson_minclus3_icl", dict(kind="modules", G=24, S=160, seed=31, nan_rate=0.10),
+     dict(clusmethod="gmm", corrmethod="pearson", crit="ICL", minclus=3, maxclus=4), 0, 120),
+    ("gmm_spearman_maxclus1", dict(kind="modules", G=24, S=120, seed=32, nan_rate=0.10),
+     dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", maxclus=1, postout=True), 0, 150),
+    ("gmm_pearson_minsamp_high", dict(kind="modules", G=24, S=140, seed=33, nan_rate=0.12),
+     dict(clusmethod="gmm", corrmethod="pearson", crit="BIC", minsamp=110), 0, 150),
+    ("gmm_spearman_S64", dict(kind="modules", G=24, S=64, seed=34, nan_rate=0.0),
+     dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", minsamp=20), 0, 120),
+    ("gmm_spearman_S65", dict(kind="modules", G=24, S=65, seed=35, nan_rate=0.0),
+     dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", minsamp=20), 0, 120),
+    ("gmm_pearson_S1024", dict(kind="modules", G=10, S=1024, seed=36, nan_rate=0.0),
+     dict(clusmethod="gmm", corrmethod="pearson", crit="BIC"), 0, 45),
+    ("gmm_pearson_S1025", dict(kind="modules", G=10, S=1025, seed=37, nan_rate=0.0),
+     dict(clusmethod="gmm", corrmethod="pearson", crit="BIC"), 0, 45),
 ]
 
 
