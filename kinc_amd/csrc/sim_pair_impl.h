@@ -845,6 +845,48 @@ __device__ __forceinline__ float diff_norm(float ax, float ay, float bx, float b
    return sqrtf(dist);
 }
 
+/* squared distance exactly as diff_norm forms it before the square root */
+__device__ __forceinline__ float diff_sq(float ax, float ay, float bx, float by)
+{
+   float dist = 0.0f;
+   dist = dist + (ax - bx) * (ax - bx);
+   dist = dist + (ay - by) * (ay - by);
+   return dist;
+}
+
+/*
+ * Nearest mean of one point in the reference's terms (pairwise_kmeans.cpp:
+ * 37-54): the first k whose sqrt(q_k) is smallest, q_k the squared distance,
+ * where a NaN distance at k = 0 wins and later NaNs never do.  sqrt is
+ * monotone and correctly rounded, so the first k with the smallest q_k (same
+ * comparisons on q) is that k unless an earlier k has a q_k so close above
+ * the minimum that both square roots round to the same float, which needs
+ * q_k < q_min * (1 + 2^-21).  Such a near tie is flagged (q_k <= q_min *
+ * (1 + 2^-20), a safe margin also in the subnormal range) and the caller then
+ * decides with the square roots.
+ */
+template <int K>
+__device__ __forceinline__ int nearest_mean_q(const float (&q)[K], bool& near_tie)
+{
+   int kq = 0;
+   float qmin = q[0];
+#pragma unroll
+   for ( int k = 1; k < K; ++k )
+   {
+      if ( q[k] < qmin )
+      {
+         kq = k;
+         qmin = q[k];
+      }
+   }
+   const float thr = qmin * (1.0f + 0x1p-20f);
+   bool tie = false;
+#pragma unroll
+   for ( int k = 0; k < K - 1; ++k ) tie = tie || (k < kq && q[k] <= thr);
+   near_tie = tie;
+   return kq;
+}
+
 /* KMeans::fit (pairwise_kmeans.cpp:12-127) with the per-fit LCG.  Means live
  * in LDS (2 floats per cluster); the best init's labels go to L.w. */
 template <class ST, int K>
@@ -899,16 +941,25 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
             {
                const float xa = st.X0(m);
                const float xb = st.X1(m);
-               int min_k = -1;
-               float min_d = 0.0f;
+               float q[K];
 #pragma unroll
-               for ( int k = 0; k < K; ++k )
+               for ( int k = 0; k < K; ++k ) q[k] = diff_sq(xa, xb, m0[k], m1[k]);
+               bool near_tie;
+               int min_k = nearest_mean_q<K>(q, near_tie);
+               if ( ballot(near_tie) != 0 )
                {
-                  const float d = diff_norm(xa, xb, m0[k], m1[k]);
-                  if ( min_k == -1 || d < min_d )
+                  /* the reference's comparison of square roots */
+                  min_k = -1;
+                  float min_d = 0.0f;
+#pragma unroll
+                  for ( int k = 0; k < K; ++k )
                   {
-                     min_k = k;
-                     min_d = d;
+                     const float d = sqrtf(q[k]);
+                     if ( min_k == -1 || d < min_d )
+                     {
+                        min_k = k;
+                        min_d = d;
+                     }
                   }
                }
                st.YN(m) = (int8_t)min_k;
