@@ -158,6 +158,7 @@ def cpu_baseline(oracle, native, x, flags, blocks, seconds, gpu_out):
     done = 0
     t_used = 0.0
     mismatches = 0
+    errs = []
     per_block = 16
     for i, b in enumerate(blocks):
         if i == 1:
@@ -172,7 +173,16 @@ def cpu_baseline(oracle, native, x, flags, blocks, seconds, gpu_out):
         got = tuple(a[i][:per_block] for a in
                     (gpu_out["K"], gpu_out["labels"], gpu_out["corr"], gpu_out["iters"]))
         mismatches += compare_block(got, want)
-    return done / t_used, done, t_used, done, mismatches
+        # the device's correlations against the same clusters' correlations
+        # in FP64 (the oracle's double-precision Pearson / exact-integer
+        # Spearman)
+        f64 = oracle.corr_fp64_block(p, x, start, per_block, got[0], got[1])
+        ok = np.isfinite(f64) & np.isfinite(got[2])
+        errs.append(np.abs(got[2][ok].astype(np.float64) - f64[ok]))
+    e = np.concatenate(errs) if errs else np.zeros(0)
+    fp64 = {"max_abs": float(e.max()) if e.size else None,
+            "mean_abs": float(e.mean()) if e.size else None, "clusters": int(e.size)}
+    return done / t_used, done, t_used, done, mismatches, fp64
 
 
 def host_threads():
@@ -350,6 +360,8 @@ def main():
                     help="time the CPU oracle on one thread only")
     ap.add_argument("--config", default="H", choices=["A", "B", "H", "C"],
                     help="workload (H is the headline; the others are reported for comparison)")
+    ap.add_argument("--pearson-fp64", action="store_true",
+                    help="Pearson sums and quotient in double (the pearson_fp64 option)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-rmt", action="store_true", help="skip the RMT threshold-sweep leg")
     ap.add_argument("--serve-queries", type=int, default=24)
@@ -388,6 +400,8 @@ def main():
 
     cfg = gemgen.CONFIGS[args.config]
     flags = dict(cfg["params"])
+    if args.pearson_fp64:
+        flags["pearson_fp64"] = True
     t = time.time()
     x = gemgen.make(args.config)
     G, S = x.shape
@@ -446,11 +460,14 @@ def main():
     whole_tflops = flops_total / secs / 1e12
 
     cpu = None
+    fp64_err = None
     if not args.no_cpu:
         import oracle
         sys.path.insert(0, os.path.join(ROOT, "tests"))
-        rate, n, used, checked, bad = cpu_baseline(oracle, native, x, flags, blocks,
-                                                   args.cpu_seconds, out)
+        rate, n, used, checked, bad, fp64_err = cpu_baseline(oracle, native, x, flags, blocks,
+                                                             args.cpu_seconds, out)
+        log("correlations vs FP64 on the sampled pairs: max %.3g, mean %.3g over %d clusters"
+            % (fp64_err["max_abs"] or 0, fp64_err["mean_abs"] or 0, fp64_err["clusters"]))
         log("cpu oracle: %d pairs in %.1fs cpu -> %.2f pairs/s; parity %d/%d mismatching"
             % (n, used, rate, bad, checked))
         cpu = {"value": rate, "unit": "pairs/s", "cores": 1, "kind": "port",
@@ -498,6 +515,7 @@ def main():
             "step": "one 32768-pair work block",
             "form": "persistent",
             "parallelism": "1 GPU",
+            "pearson_fp64": bool(args.pearson_fp64),
         },
         "plain_value": plain_value,
         "batched_vs_plain": (value / plain_value) if plain_value else None,
@@ -508,6 +526,7 @@ def main():
             "whole_job_tflops": whole_tflops,
         }),
         "cpu_baseline": cpu,
+        "corr_error_vs_fp64": fp64_err,
         "serving": serving_obj,
         "rmt": rmt_obj,
     }

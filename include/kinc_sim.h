@@ -77,7 +77,13 @@ typedef struct kinc_sim_params
    int32_t ksize;          /* --ksize, pairs per plain-form launch, default 4096 */
    int32_t form;           /* KINC_FORM_*, default persistent */
    int32_t batch_capacity; /* largest count of one execute call; 0 = 32768 */
-   int32_t reserved[4];    /* must be zero */
+   int32_t pearson_fp64;   /* 0 (default): Pearson moments summed as the reference
+                              sums them (plain FP32, pairwise_pearson.cpp:26-48);
+                              1: the same sums in FP64 over exact products and the
+                              quotient in FP64, rounded to float: an extension for
+                              callers who want correlations within half an ulp of
+                              the exact value (K, labels and Spearman unchanged) */
+   int32_t reserved[3];    /* must be zero */
 } kinc_sim_params;
 
 typedef struct kinc_sim kinc_sim;

@@ -44,7 +44,8 @@ class SimParams(ctypes.Structure):
         ("ksize", ctypes.c_int32),
         ("form", ctypes.c_int32),
         ("batch_capacity", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("pearson_fp64", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -97,7 +98,7 @@ def default_params():
 
 def make_params(clusmethod="none", corrmethod="pearson", minexpr=float("-inf"), minsamp=30,
                 minclus=1, maxclus=5, crit="ICL", preout=False, postout=False, mincorr=0.5,
-                maxcorr=1.0, ksize=4096, form="persistent", batch_capacity=0):
+                maxcorr=1.0, ksize=4096, form="persistent", batch_capacity=0, pearson_fp64=False):
     """Build kinc_sim_params from the reference's CLI flag names and defaults
     (src/core/similarity_input.cpp:96-254)."""
     p = default_params()
@@ -115,6 +116,7 @@ def make_params(clusmethod="none", corrmethod="pearson", minexpr=float("-inf"), 
     p.ksize = ksize
     p.form = FORM[form]
     p.batch_capacity = batch_capacity
+    p.pearson_fp64 = int(bool(pearson_fp64))
     return p
 
 

@@ -9,7 +9,8 @@
  *        [--minexpr F] [--minsamp N] [--minclus N] [--maxclus N]
  *        [--crit BIC|ICL] [--preout true|false] [--postout true|false]
  *        [--mincorr F] [--maxcorr F] [--ksize N]
- * plus this build's --device N and --form plain|persistent; and the
+ * plus this build's --device N, --form plain|persistent and --pearson-fp64
+ * BOOL (Pearson sums and quotient in double); and the
  * expression-matrix converters around it (importexpressionmatrix_input.cpp,
  * exportexpressionmatrix_input.cpp):
  *   kinc run import-emx --input A.txt --output A.emx [--nan TOKEN]
@@ -57,6 +58,7 @@ int usage()
            "       [--corrmethod pearson|spearman] [--minexpr F] [--minsamp N] [--minclus N]\n"
            "       [--maxclus N] [--crit BIC|ICL] [--preout BOOL] [--postout BOOL]\n"
            "       [--mincorr F] [--maxcorr F] [--ksize N] [--device N] [--form plain|persistent]\n"
+           "       [--pearson-fp64 BOOL]\n"
            "       kinc run import-emx --input TXT --output EMX [--nan TOKEN] [--samples N]\n"
            "       [--transform none|'natural logarithm'|'logarithm base 2'|'logarithm base 10']\n"
            "       [--threads N]\n"
@@ -246,6 +248,7 @@ int main(int argc, char** argv)
       else if ( k == "--ksize" ) p.ksize = atoi(v.c_str());
       else if ( k == "--device" ) device = atoi(v.c_str());
       else if ( k == "--form" ) p.form = v == "plain" ? KINC_FORM_PLAIN : KINC_FORM_PERSISTENT;
+      else if ( k == "--pearson-fp64" ) p.pearson_fp64 = parse_bool(v);
       else return usage();
    }
    if ( in.empty() || ccm.empty() || cmx.empty() )

@@ -208,6 +208,7 @@ static int validate(const kinc_sim_params* p, int G, int S, std::string& why)
       return KINC_E_INVALID;
    }
    if ( p->form != KINC_FORM_PLAIN && p->form != KINC_FORM_PERSISTENT ) { why = "form must be 0 (plain) or 1 (persistent)"; return KINC_E_INVALID; }
+   if ( p->pearson_fp64 != 0 && p->pearson_fp64 != 1 ) { why = "pearson_fp64 must be 0 or 1"; return KINC_E_INVALID; }
    for ( int r : p->reserved )
    {
       if ( r != 0 ) { why = "reserved fields must be zero"; return KINC_E_INVALID; }
@@ -311,6 +312,7 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    a.crit = params->criterion;
    a.pre_out = params->pre_out ? 1 : 0;
    a.post_out = params->post_out ? 1 : 0;
+   a.pearson_fp64 = params->pearson_fp64;
    a.logN = h->d_logN;
    a.blockN = h->d_blockN;
    a.chunkMax = h->d_chunkMax;

@@ -47,6 +47,7 @@ struct SimArgs
    int32_t crit;
    int32_t pre_out;
    int32_t post_out;
+   int32_t pearson_fp64;     /* Pearson sums and quotient in double */
 
    const int2* pairs;        /* explicit (x, y) of each block-relative pair, or null
                                 for linear indices from block_start */

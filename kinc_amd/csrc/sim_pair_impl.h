@@ -75,6 +75,14 @@ __device__ __forceinline__ float bcast(float v, int src)
    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
+__device__ __forceinline__ double bcast_d(double v, int src)
+{
+   const uint64_t b = (uint64_t)__double_as_longlong(v);
+   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 __device__ __forceinline__ int bcast_i(int v, int src)
 {
    return __builtin_amdgcn_readlane(v, src);
@@ -1176,7 +1184,9 @@ __device__ void fill_stale(const SimArgs& a, const Lds& L, int q, int N)
 /* -------------------------------------------------------------- Pearson */
 
 /* pairwise_pearson.cpp:12-52 for clusters 0..K-1 at once: chain 5k+t holds
- * sum x, sum y, sum x^2, sum y^2, sum xy of cluster k. */
+ * sum x, sum y, sum x^2, sum y^2, sum xy of cluster k.  With pearson_fp64
+ * (an option the reference lacks) the same sums run in double over exact
+ * products and the quotient is evaluated in double, then rounded to float. */
 template <int KMAX>
 __device__ void pearson_all(const SimArgs& a, const Lds& L, int K, float* corr)
 {
@@ -1184,21 +1194,38 @@ __device__ void pearson_all(const SimArgs& a, const Lds& L, int K, float* corr)
    const int lane = lane_id();
    const int nch = 5 * K;
    float acc = 0.0f;
+   double dacc = 0.0;   /* pearson_fp64 mode */
    /* lanes nch..15 (cluster index >= K, which no label takes) add nothing */
    if ( lane < nch || lane < KINC_MIN_ACTIVE_LANES )
    {
       const int k = lane / 5;
       const int t = lane - 5 * k;
-      for ( int i = 0; i < S; ++i )
+      if ( !a.pearson_fp64 )
       {
-         int8_t l = L.lab[i];
-         float2 v = L.X[i];
-         float term = (t == 0) ? v.x
-                    : (t == 1) ? v.y
-                    : (t == 2) ? v.x * v.x
-                    : (t == 3) ? v.y * v.y
-                    : v.x * v.y;
-         acc = (l == k) ? acc + term : acc;
+         for ( int i = 0; i < S; ++i )
+         {
+            int8_t l = L.lab[i];
+            float2 v = L.X[i];
+            float term = (t == 0) ? v.x
+                       : (t == 1) ? v.y
+                       : (t == 2) ? v.x * v.x
+                       : (t == 3) ? v.y * v.y
+                       : v.x * v.y;
+            acc = (l == k) ? acc + term : acc;
+         }
+      }
+      else
+      {
+         /* term = f1 * f2, exact in double (two 24-bit significands) */
+         for ( int i = 0; i < S; ++i )
+         {
+            int8_t l = L.lab[i];
+            float2 v = L.X[i];
+            const float f1 = (t == 1 || t == 3) ? v.y : v.x;
+            const float f2 = (t < 2) ? 1.0f : (t == 2) ? v.x : v.y;
+            const double term = (double)f1 * (double)f2;
+            dacc = (l == k) ? dacc + term : dacc;
+         }
       }
    }
 #pragma unroll
@@ -1213,13 +1240,21 @@ __device__ void pearson_all(const SimArgs& a, const Lds& L, int K, float* corr)
             n += popc(ballot(i < S && L.lab[i] == k));
          }
          float r = NAN;
-         if ( n >= a.min_samples )
+         if ( n >= a.min_samples && !a.pearson_fp64 )
          {
             const float fn = (float)n;
             float sx = bcast(acc, 5 * k), sy = bcast(acc, 5 * k + 1);
             float sxx = bcast(acc, 5 * k + 2), syy = bcast(acc, 5 * k + 3);
             float sxy = bcast(acc, 5 * k + 4);
             r = (fn * sxy - sx * sy) / sqrtf((fn * sxx - sx * sx) * (fn * syy - sy * sy));
+         }
+         else if ( n >= a.min_samples )
+         {
+            const double dn = (double)n;
+            const double sx = bcast_d(dacc, 5 * k), sy = bcast_d(dacc, 5 * k + 1);
+            const double sxx = bcast_d(dacc, 5 * k + 2), syy = bcast_d(dacc, 5 * k + 3);
+            const double sxy = bcast_d(dacc, 5 * k + 4);
+            r = (float)((dn * sxy - sx * sy) / sqrt((dn * sxx - sx * sx) * (dn * syy - sy * sy)));
          }
          corr[k] = r;
       }

@@ -89,9 +89,10 @@ def corr_fp64_block(params, emx, start, count, K, labels):
     emx = np.ascontiguousarray(emx, np.float32)
     G, S = emx.shape
     out = np.empty((count, params.max_clusters), np.float64)
+    K = np.ascontiguousarray(K, np.int8)
+    labels = np.ascontiguousarray(labels, np.int8)
     lib().oracle_corr_fp64_block(ctypes.byref(params), emx.ctypes.data, G, S, start, count,
-                                 np.ascontiguousarray(K).ctypes.data,
-                                 np.ascontiguousarray(labels).ctypes.data, out.ctypes.data)
+                                 K.ctypes.data, labels.ctypes.data, out.ctypes.data)
     return out
 
 

@@ -619,6 +619,30 @@ static int clustering_compute(const kinc_sim_params* p, const vec2* X, int N,
 
 /* src/core/pairwise_pearson.cpp:12-52: full-length loop over labels with
  * data[i] read at the full index. */
+/* kinc_sim_params.pearson_fp64 = 1 (an option the reference lacks): the
+ * same sums in the same order, in double over exact products, and the
+ * quotient in double, rounded to float at the end. */
+static float pearson_fp64(const vec2* data, const int8_t* labels, int S,
+                          int8_t cluster, int minSamples)
+{
+   int n = 0;
+   double sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0;
+   for ( int i = 0; i < S; ++i )
+   {
+      if ( labels[i] != cluster ) continue;
+      const double x_i = data[i].s[0];
+      const double y_i = data[i].s[1];
+      sx += x_i;
+      sy += y_i;
+      sxx += x_i * x_i;
+      syy += y_i * y_i;
+      sxy += x_i * y_i;
+      ++n;
+   }
+   if ( n < minSamples ) return NAN;
+   return (float)((n * sxy - sx * sy) / sqrt((n * sxx - sx * sx) * (n * syy - sy * sy)));
+}
+
 float oracle_pearson(const vec2* data, const int8_t* labels, int S,
                      int8_t cluster, int minSamples)
 {
@@ -781,7 +805,8 @@ int oracle_execute_block(const kinc_sim_params* p, const float* emx,
       for ( int k = 0; k < K; ++k )
       {
          corr[k] = (p->corr_method == KINC_CORR_PEARSON)
-            ? oracle_pearson(X, labels, S, (int8_t)k, p->min_samples)
+            ? (p->pearson_fp64 ? pearson_fp64(X, labels, S, (int8_t)k, p->min_samples)
+                               : oracle_pearson(X, labels, S, (int8_t)k, p->min_samples))
             : oracle_spearman(X, labels, S, (int8_t)k, p->min_samples, wx, wy, wr);
       }
 
@@ -840,7 +865,8 @@ int oracle_execute_pairs(const kinc_sim_params* p, const float* emx, int32_t G, 
       for ( int k = 0; k < K; ++k )
       {
          corr[k] = (p->corr_method == KINC_CORR_PEARSON)
-            ? oracle_pearson(X, labels, S, (int8_t)k, p->min_samples)
+            ? (p->pearson_fp64 ? pearson_fp64(X, labels, S, (int8_t)k, p->min_samples)
+                               : oracle_pearson(X, labels, S, (int8_t)k, p->min_samples))
             : oracle_spearman(X, labels, S, (int8_t)k, p->min_samples, wx, wy, wr);
       }
       K_out[q] = (int8_t)K;

@@ -62,6 +62,14 @@ CASES = [
      dict(clusmethod="gmm", corrmethod="pearson", crit="BIC"), 0, 45),
     ("gmm_pearson_S1025", dict(kind="modules", G=10, S=1025, seed=37, nan_rate=0.0),
      dict(clusmethod="gmm", corrmethod="pearson", crit="BIC"), 0, 45),
+    # the FP64 Pearson option: double sums and quotient on both sides
+    ("pearson_none_fp64", dict(kind="latent", G=100, S=300, seed=38, nan_rate=0.05),
+     dict(clusmethod="none", corrmethod="pearson", pearson_fp64=True), 0, 4950),
+    ("gmm_pearson_fp64_out", dict(kind="modules", G=30, S=220, seed=39, nan_rate=0.10),
+     dict(clusmethod="gmm", corrmethod="pearson", crit="BIC", preout=True, postout=True,
+          pearson_fp64=True), 0, 200),
+    ("gmm_pearson_fp64_S1100", dict(kind="modules", G=10, S=1100, seed=40, nan_rate=0.05),
+     dict(clusmethod="gmm", corrmethod="pearson", crit="ICL", pearson_fp64=True), 0, 45),
 ]
 
 
