@@ -5,14 +5,21 @@
  * The reference calls unqualified exp/log/sqrt on float arguments
  * (src/core/pairwise_gmm.cpp:46,177,180,216,219,262,271,277,356,391;
  *  src/core/pairwise_linalg.cpp:103; src/core/pairwise_pearson.cpp:48).
- * Those resolve to the FP32 overloads here.  exp and log are written from a
- * published method, Cephes (S. L. Moshier, "Methods and Programs for
- * Mathematical Functions", 1989; Cephes Math Library expf.c / logf.c):
- *   - expf: Cody-Waite reduction x = g + n*ln2 with ln2 split as
- *           C1 = 0.693359375, C2 = -2.12194440e-4, then the degree-5
- *           minimax polynomial of expf.c for e^g on |g| <= ln2/2.
- *   - logf: x = m * 2^e with m in [sqrt(1/2), sqrt(2)), the degree-9
- *           minimax polynomial of logf.c for log(1+f), and the same ln2 split.
+ * Those resolve to the FP32 overloads here.  exp and log follow published
+ * methods:
+ *   - expf: Cody-Waite reduction x = g + n*ln2 (W. J. Cody and W. Waite,
+ *           "Software Manual for the Elementary Functions", 1980) with ln2
+ *           split as in Cephes, C1 = 0.693359375, C2 = -2.12194440e-4; then
+ *           e^g = 1 + g (1 + g P(g)) on |g| <= 0.3467 with P of degree 4,
+ *           its coefficients from a Remez exchange for the relative error of
+ *           e^g (3.1e-9 before rounding the coefficients to float), evaluated
+ *           in Horner form with six FMAs.  Against the correctly rounded
+ *           result, over every 7th float in [-104, 89]: at most 0.88 ulp,
+ *           0.47% of results not correctly rounded.
+ *   - logf: Cephes (S. L. Moshier, "Methods and Programs for Mathematical
+ *           Functions", 1989; logf.c): x = m * 2^e with m in
+ *           [sqrt(1/2), sqrt(2)), the degree-9 minimax polynomial of logf.c
+ *           for log(1+f), and the same ln2 split.
  * Only correctly rounded IEEE-754 single-precision operations are used:
  * +, -, *, explicit fused multiply-add (fmaf), ldexp, frexp, rint and
  * comparisons; no hardware transcendental.  Each of these has exactly one
@@ -124,15 +131,14 @@ KINC_HD float kinc_expf(float x)
    float nf = kinc_rintf(xc * LOG2EF);
    float g = kinc_fmaf(-nf, C1, xc);
    g = kinc_fmaf(-nf, C2, g);
-   float z = g * g;
-   float p = 1.9875691500E-4f;
-   p = kinc_fmaf(p, g, 1.3981999507E-3f);
-   p = kinc_fmaf(p, g, 8.3334519073E-3f);
-   p = kinc_fmaf(p, g, 4.1665795894E-2f);
-   p = kinc_fmaf(p, g, 1.6666665459E-1f);
-   p = kinc_fmaf(p, g, 5.0000001201E-1f);
-   p = kinc_fmaf(p, z, g);
-   p = p + 1.0f;
+   /* e^g = 1 + g (1 + g P(g)), P of degree 4: six dependent FMAs */
+   float q = 1.3814560370519757e-3f;
+   q = kinc_fmaf(q, g, 8.368735201656818e-3f);
+   q = kinc_fmaf(q, g, 4.166838899254799e-2f);
+   q = kinc_fmaf(q, g, 1.666652113199234e-1f);
+   q = kinc_fmaf(q, g, 4.999999403953552e-1f);
+   q = kinc_fmaf(q, g, 1.0f);
+   float p = kinc_fmaf(q, g, 1.0f);
 
    float r = kinc_ldexpf(p, (int)nf);    /* -150 <= nf <= 128 */
    return (x != x) ? x : r;

@@ -79,7 +79,13 @@ def test_expf_accuracy_and_specials():
     got = np.array([oracle.expf(float(v)) for v in xs], np.float32)
     want = np.exp(xs.astype(np.float64)).astype(np.float32)
     normal = want > np.float32(1.2e-38)
-    assert ulp_diff(got[normal], want[normal]).max() <= 2
+    # want is the correctly rounded result (double exp rounded once): the
+    # polynomial's error stays below one ulp (0.88 at worst over every 7th
+    # float of [-104, 89]), so results are correct or one ulp off, and rarely
+    # the latter
+    d = ulp_diff(got[normal], want[normal])
+    assert d.max() <= 1
+    assert (d != 0).mean() < 0.01
     # denormal results stay within a few units of the smallest denormal
     sub = ~normal
     assert np.all(np.abs(got[sub].astype(np.float64) - np.exp(xs[sub].astype(np.float64)))
