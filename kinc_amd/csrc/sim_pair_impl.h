@@ -178,13 +178,14 @@ struct Lds
  * in the wave's slice of a device scratch buffer and only the parameters and
  * the chain tile in LDS.
  */
+template <bool MEM>
 __device__ __forceinline__ Lds lds_view(const SimArgs& a)
 {
    extern __shared__ __align__(16) char lds_raw[];
    Lds l;
    l.prm = (float*)(lds_raw + a.lds_prm);
    l.tile = (float*)(lds_raw + a.lds_tile);
-   if ( a.scratch == nullptr )
+   if ( !MEM )
    {
       l.X = (float2*)(lds_raw + a.lds_x);
       l.lab = (int8_t*)(lds_raw + a.lds_lab);
@@ -209,7 +210,7 @@ __device__ __forceinline__ Lds lds_view(const SimArgs& a)
 /* ---------------------------------------------------------------- fetch */
 
 /* similarity_serial.cpp:108-141: labels -9/-6/0 and compaction in sample order. */
-__device__ int fetch_pair(const SimArgs& a, int gx, int gy, const Lds& L)
+__device__ __forceinline__ int fetch_pair(const SimArgs& a, int gx, int gy, const Lds& L)
 {
    const int S = a.S;
    const float* r1 = a.emx + (size_t)gx * (size_t)S;
@@ -252,7 +253,7 @@ __device__ int fetch_pair(const SimArgs& a, int gx, int gy, const Lds& L)
  * across lanes gives the same result as the serial loop.
  */
 template <bool PAYLOAD>
-__device__ void bitonic_sort(float* key, float* pay, int size, bool global_mem)
+__device__ __forceinline__ void bitonic_sort(float* key, float* pay, int size, bool global_mem)
 {
    const int lane = lane_id();
    const int half = size >> 1;
@@ -290,7 +291,7 @@ __device__ void bitonic_sort(float* key, float* pay, int size, bool global_mem)
 /* ------------------------------------------------------------- outliers */
 
 /* pairwise_clustering.cpp:110-148 */
-__device__ void mark_outliers(const Lds& L, int N, int j, int8_t cluster, int8_t marker)
+__device__ __forceinline__ void mark_outliers(const Lds& L, int N, int j, int8_t cluster, int8_t marker)
 {
    const int lane = lane_id();
    float* key = L.u;
@@ -341,7 +342,7 @@ __device__ void mark_outliers(const Lds& L, int N, int j, int8_t cluster, int8_t
 
 /* pairwise_clustering.cpp:77-84: labels[i] >= 0 (i < N) take the fit's
  * labels in compact order. */
-__device__ void copy_back(const Lds& L, int N)
+__device__ __forceinline__ void copy_back(const Lds& L, int N)
 {
    const int lane = lane_id();
    int jb = 0;
@@ -1142,7 +1143,7 @@ __device__ int find_prev_greater(const SimArgs& a, int qp, int v, int lo)
  * i >= N of pair q therefore holds entry i of the latest earlier pair of the
  * block with more than i valid samples, or zero.  Rebuild that view in L.X.
  */
-__device__ void fill_stale(const SimArgs& a, const Lds& L, int q, int N)
+__device__ __forceinline__ void fill_stale(const SimArgs& a, const Lds& L, int q, int N)
 {
    const int S = a.S;
    const int lane = lane_id();
@@ -1188,7 +1189,7 @@ __device__ void fill_stale(const SimArgs& a, const Lds& L, int q, int N)
  * (an option the reference lacks) the same sums run in double over exact
  * products and the quotient is evaluated in double, then rounded to float. */
 template <int KMAX>
-__device__ void pearson_all(const SimArgs& a, const Lds& L, int K, float* corr)
+__device__ __forceinline__ void pearson_all(const SimArgs& a, const Lds& L, int K, float* corr)
 {
    const int S = a.S;
    const int lane = lane_id();
@@ -1264,7 +1265,7 @@ __device__ void pearson_all(const SimArgs& a, const Lds& L, int K, float* corr)
 /* ------------------------------------------------------------- Spearman */
 
 /* pairwise_spearman.cpp:27-86 */
-__device__ float spearman_k(const SimArgs& a, const Lds& L, int k)
+__device__ __forceinline__ float spearman_k(const SimArgs& a, const Lds& L, int k)
 {
    const int S = a.S;
    const int lane = lane_id();
@@ -1382,7 +1383,7 @@ __device__ __forceinline__ FitOut fit_dispatch(const SimArgs& a, const Lds& L, i
 template <int P, int KMAX>
 __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
 {
-   const Lds L = lds_view(a);
+   const Lds L = lds_view<P == 0>(a);
    const int lane = lane_id();
    const int S = a.S;
    const int MC = a.max_k;
