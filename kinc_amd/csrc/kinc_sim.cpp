@@ -33,6 +33,7 @@ KINC_FOR_EACH_INSTANCE(KINC_DECL)
 #undef KINC_DECL
 const void* kinc_kernel_count_valid();
 const void* kinc_kernel_chunk_max();
+const void* kinc_kernel_prev_greater();
 const void* kinc_kernel_bin_pairs();
 const void* kinc_kernel_zero_i32();
 }
@@ -105,6 +106,8 @@ struct kinc_sim
    double* d_logN = nullptr;
    int32_t* d_blockN = nullptr;
    int32_t* d_chunkMax = nullptr;
+   float* d_xbuf = nullptr;        /* Pearson, register-slot layout: compact X per pair */
+   int32_t* d_prev = nullptr;      /* with d_xbuf: previous-greater link per pair */
    int32_t* d_counters = nullptr;  /* [2*kMaxBins]: bin sizes, then pop heads */
    int32_t* d_lists = nullptr;     /* [kMaxBins][KINC_WORK_BLOCK_SIZE] */
    int32_t* d_binP = nullptr;
@@ -319,6 +322,16 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    a.np2 = next_pow2(S);
    a.tile_stride = KINC_TILE_STRIDE;
    a.mem_slots = (S + 63) / 64;
+   if ( params->corr_method == KINC_CORR_PEARSON && h->classes[0] != 0 )
+   {
+      /* Pearson's stale reads copy earlier pairs' compact X from here
+       * instead of fetching and compacting their rows again (at most
+       * 32768 x 1024 x 8 bytes = 256 MiB) */
+      CREATE_TRY(hipMalloc(&h->d_xbuf, sizeof(float) * 2 * B * (size_t)S));
+      CREATE_TRY(hipMalloc(&h->d_prev, sizeof(int32_t) * B));
+      a.xbuf = h->d_xbuf;
+      a.prev = h->d_prev;
+   }
    const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
    const size_t sort_bytes = sizeof(float) * 2 * (size_t)a.np2;
 
@@ -488,7 +501,8 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       float me = h->p.min_expr;
       int32_t* outN = h->d_blockN;
       int G = h->G;
-      void* args[] = {&emx, &S, &G, &bs, &pl, (void*)&cnt, &me, &outN};
+      float* xb = h->d_xbuf;
+      void* args[] = {&emx, &S, &G, &bs, &pl, (void*)&cnt, &me, &outN, &xb};
       int rc = launch(h, kinc_kernel_count_valid(), (unsigned)((cnt + 3) / 4), 256, args, 0, s);
       if ( rc ) return rc;
    }
@@ -500,6 +514,14 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       const int chunks = (cnt + 63) / 64;
       int rc = launch(h, kinc_kernel_chunk_max(), (unsigned)((chunks + 255) / 256), 256, args, 0, s);
       if ( rc ) return rc;
+      if ( h->d_prev )
+      {
+         const int32_t* sl = seg_lo;
+         int32_t* pv = h->d_prev;
+         void* pargs[] = {&Np, &cm, &sl, (void*)&cnt, &pv};
+         rc = launch(h, kinc_kernel_prev_greater(), (unsigned)((cnt + 3) / 4), 256, pargs, 0, s);
+         if ( rc ) return rc;
+      }
    }
 
    if ( h->p.form == KINC_FORM_PLAIN )
@@ -691,6 +713,8 @@ extern "C" void kinc_sim_destroy(kinc_sim* h)
    (void)hipFree(h->d_logN);
    (void)hipFree(h->d_blockN);
    (void)hipFree(h->d_chunkMax);
+   (void)hipFree(h->d_xbuf);
+   (void)hipFree(h->d_prev);
    (void)hipFree(h->d_counters);
    (void)hipFree(h->d_lists);
    (void)hipFree(h->d_binP);

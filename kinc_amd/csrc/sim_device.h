@@ -56,6 +56,10 @@ struct SimArgs
    const double* logN;       /* log((double)n) for n in [0, S] */
    const int32_t* blockN;    /* valid-sample count of every pair of the block */
    const int32_t* chunkMax;  /* max of blockN over 64-pair chunks */
+   const float* xbuf;        /* [block_count][S] float pairs: each pair's compact X,
+                                written by the count kernel, or null */
+   const int32_t* prev;      /* [block_count] latest earlier pair of the segment with
+                                more valid samples (-1: none), with xbuf */
 
    int8_t* K_out;            /* [block_count] */
    int8_t* labels_out;       /* [block_count][S] or null */

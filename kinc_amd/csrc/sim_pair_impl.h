@@ -217,27 +217,39 @@ __device__ __forceinline__ int fetch_pair(const SimArgs& a, int gx, int gy, cons
    const float* r2 = a.emx + (size_t)gy * (size_t)S;
    const int lane = lane_id();
    int base = 0;
-   for ( int s0 = 0; s0 < S; s0 += WAVE )
+   /* four slots' loads are issued before any of them is used */
+   for ( int s0 = 0; s0 < S; s0 += 4 * WAVE )
    {
-      int s = s0 + lane;
-      bool in = s < S;
-      float g1 = in ? r1[s] : 0.0f;
-      float g2 = in ? r2[s] : 0.0f;
-      int8_t lab = 0;
-      bool valid = false;
-      if ( in )
+      float g1[4], g2[4];
+#pragma unroll
+      for ( int j = 0; j < 4; ++j )
       {
-         if ( isnan(g1) || isnan(g2) ) lab = -9;
-         else if ( g1 < a.min_expr || g2 < a.min_expr ) lab = -6;
-         else valid = true;
-         L.lab[s] = lab;
+         const int s = s0 + j * WAVE + lane;
+         const bool in = s < S;
+         g1[j] = in ? r1[s] : 0.0f;
+         g2[j] = in ? r2[s] : 0.0f;
       }
-      uint64_t m = ballot(valid);
-      if ( valid )
+#pragma unroll
+      for ( int j = 0; j < 4; ++j )
       {
-         L.X[base + popc(m & lanes_below())] = make_float2(g1, g2);
+         const int s = s0 + j * WAVE + lane;
+         const bool in = s < S;
+         int8_t lab = 0;
+         bool valid = false;
+         if ( in )
+         {
+            if ( isnan(g1[j]) || isnan(g2[j]) ) lab = -9;
+            else if ( g1[j] < a.min_expr || g2[j] < a.min_expr ) lab = -6;
+            else valid = true;
+            L.lab[s] = lab;
+         }
+         uint64_t m = ballot(valid);
+         if ( valid )
+         {
+            L.X[base + popc(m & lanes_below())] = make_float2(g1[j], g2[j]);
+         }
+         base += popc(m);
       }
-      base += popc(m);
    }
    wave_sync(L.lgbuf != nullptr);
    return base;
@@ -1100,14 +1112,15 @@ __device__ __forceinline__ void pair_genes(const SimArgs& a, int q, int& x, int&
 
 /* Last block-relative pair d with lo <= d < qp and blockN[d] > v, or -1.
  * Chunks of 64 pairs lying wholly in [lo, qp) are skipped by their maxima. */
-__device__ int find_prev_greater(const SimArgs& a, int qp, int v, int lo)
+__device__ __forceinline__ int prev_greater(const int32_t* blockN, const int32_t* chunkMax,
+                                            int qp, int v, int lo)
 {
    const int lane = lane_id();
    if ( qp <= lo ) return -1;
    const int c = (qp - 1) >> 6;
    {
       int idx = c * WAVE + lane;
-      bool ok = idx >= lo && idx < qp && a.blockN[idx] > v;
+      bool ok = idx >= lo && idx < qp && blockN[idx] > v;
       uint64_t m = ballot(ok);
       if ( m ) return c * WAVE + 63 - __clzll(m);
    }
@@ -1115,13 +1128,13 @@ __device__ int find_prev_greater(const SimArgs& a, int qp, int v, int lo)
    for ( int cb = c - 1; cb >= cfull; cb -= WAVE )
    {
       int cc = cb - lane;
-      bool ok = cc >= cfull && a.chunkMax[cc] > v;
+      bool ok = cc >= cfull && chunkMax[cc] > v;
       uint64_t m = ballot(ok);
       if ( m )
       {
          int ch = cb - (__ffsll((unsigned long long)m) - 1);
          int idx = ch * WAVE + lane;
-         uint64_t m2 = ballot(a.blockN[idx] > v);
+         uint64_t m2 = ballot(blockN[idx] > v);
          return ch * WAVE + 63 - __clzll(m2);
       }
    }
@@ -1129,11 +1142,16 @@ __device__ int find_prev_greater(const SimArgs& a, int qp, int v, int lo)
    if ( cpart < cfull && cpart < c )
    {
       int idx = cpart * WAVE + lane;
-      bool ok = idx >= lo && a.blockN[idx] > v;
+      bool ok = idx >= lo && blockN[idx] > v;
       uint64_t m = ballot(ok);
       if ( m ) return cpart * WAVE + 63 - __clzll(m);
    }
    return -1;
+}
+
+__device__ __forceinline__ int find_prev_greater(const SimArgs& a, int qp, int v, int lo)
+{
+   return prev_greater(a.blockN, a.chunkMax, qp, v, lo);
 }
 
 /*
@@ -1147,6 +1165,31 @@ __device__ __forceinline__ void fill_stale(const SimArgs& a, const Lds& L, int q
 {
    const int S = a.S;
    const int lane = lane_id();
+   if ( a.xbuf != nullptr )
+   {
+      /* the loop below with its searches precomputed: entries [v, N_d)
+       * come from the compact X of d (stored by the count kernel), and the
+       * search that follows, from d with threshold N_d, is prev[d] */
+      const float2* xb = (const float2*)a.xbuf;
+      int v = N;
+      int d = a.prev[q];
+      while ( v < S )
+      {
+         if ( d < 0 )
+         {
+            for ( int i = v + lane; i < S; i += WAVE ) L.X[i] = make_float2(0.0f, 0.0f);
+            break;
+         }
+         const int hi = a.blockN[d];
+         const int dn = a.prev[d];
+         const float2* src = xb + (size_t)d * (size_t)S;
+         for ( int i = v + lane; i < hi; i += WAVE ) L.X[i] = src[i];
+         v = hi;
+         d = dn;
+      }
+      wave_sync(L.lgbuf != nullptr);
+      return;
+   }
    const int lo = a.seg_lo ? a.seg_lo[q] : 0;
    int v = N;
    int qp = q;

@@ -1,14 +1,19 @@
 /*
  * sim_util_kernels.hip — block preparation kernels of the pairwise
- * Similarity path: valid-sample counts, chunk maxima for the stale-read
- * view, sample-count binning for the persistent form, and a zero fill.
+ * Similarity path: valid-sample counts and compact X, chunk maxima and
+ * previous-greater links for the stale-read view, sample-count binning for
+ * the persistent form, and a zero fill.
  */
 #include "sim_pair_impl.h"
 
-/* Valid-sample count per pair of a block (one wave per pair, 4 per group). */
+/* Valid-sample count per pair of a block (one wave per pair, 4 per group),
+ * and with xbuf non-null the pair's compact X (fetchPair's X,
+ * similarity_serial.cpp:108-141) at xbuf[w][0..N), for the stale-read view
+ * of later pairs. */
 __global__ __launch_bounds__(256) void k_count_valid(const float* emx, int S, int G,
                                                      int64_t block_start, const int2* pairs,
-                                                     int count, float min_expr, int32_t* outN)
+                                                     int count, float min_expr, int32_t* outN,
+                                                     float* xbuf)
 {
    const int w = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
    if ( w >= count ) return;
@@ -26,19 +31,45 @@ __global__ __launch_bounds__(256) void k_count_valid(const float* emx, int S, in
    }
    const float* r1 = emx + (size_t)gx * (size_t)S;
    const float* r2 = emx + (size_t)gy * (size_t)S;
+   float2* xo = xbuf ? (float2*)xbuf + (size_t)w * (size_t)S : nullptr;
+   const int lane = lane_id();
    int n = 0;
-   for ( int s0 = 0; s0 < S; s0 += WAVE )
+   for ( int s0 = 0; s0 < S; s0 += 4 * WAVE )
    {
-      int s = s0 + lane_id();
-      bool valid = false;
-      if ( s < S )
+      float g1[4], g2[4];
+#pragma unroll
+      for ( int j = 0; j < 4; ++j )
       {
-         float g1 = r1[s], g2 = r2[s];
-         valid = !isnan(g1) && !isnan(g2) && !(g1 < min_expr || g2 < min_expr);
+         const int s = s0 + j * WAVE + lane;
+         const bool in = s < S;
+         g1[j] = in ? r1[s] : 0.0f;
+         g2[j] = in ? r2[s] : 0.0f;
       }
-      n += popc(ballot(valid));
+#pragma unroll
+      for ( int j = 0; j < 4; ++j )
+      {
+         const int s = s0 + j * WAVE + lane;
+         const bool valid = s < S && !isnan(g1[j]) && !isnan(g2[j]) &&
+                            !(g1[j] < min_expr || g2[j] < min_expr);
+         const uint64_t m = ballot(valid);
+         if ( xo && valid ) xo[n + popc(m & lanes_below())] = make_float2(g1[j], g2[j]);
+         n += popc(m);
+      }
    }
-   if ( lane_id() == 0 ) outN[w] = n;
+   if ( lane == 0 ) outN[w] = n;
+}
+
+/* prev[q]: the latest earlier pair of q's segment with more valid samples
+ * than q, or -1 (one wave per pair, 4 per group; after k_chunk_max). */
+__global__ __launch_bounds__(256) void k_prev_greater(const int32_t* N, const int32_t* cmax,
+                                                      const int32_t* seg_lo, int count,
+                                                      int32_t* prev)
+{
+   const int q = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+   if ( q >= count ) return;
+   const int lo = seg_lo ? seg_lo[q] : 0;
+   const int d = prev_greater(N, cmax, q, N[q], lo);
+   if ( lane_id() == 0 ) prev[q] = d;
 }
 
 /* max of N over 64-pair chunks */
@@ -90,6 +121,7 @@ __global__ void k_zero_i32(int32_t* p, int n)
 extern "C" {
 const void* kinc_kernel_count_valid() { return (const void*)&k_count_valid; }
 const void* kinc_kernel_chunk_max() { return (const void*)&k_chunk_max; }
+const void* kinc_kernel_prev_greater() { return (const void*)&k_prev_greater; }
 const void* kinc_kernel_bin_pairs() { return (const void*)&k_bin_pairs; }
 const void* kinc_kernel_zero_i32() { return (const void*)&k_zero_i32; }
 }
