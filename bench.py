@@ -49,7 +49,7 @@ WORKLOADS = {
 # HBM bytes per launch of the dominant pair kernel, from the FETCH_SIZE and
 # WRITE_SIZE passes (scaled by a copy kernel of known size) recorded in:
 TRAFFIC_SOURCE = "profiles/k_pairs_gmm_spearman_20000x500_counters.md"
-TRAFFIC_BYTES_PER_LAUNCH = 2466e6
+TRAFFIC_BYTES_PER_LAUNCH = 634.4e6
 BLOCK = 32768
 
 
