@@ -520,7 +520,8 @@ def main():
         "plain_value": plain_value,
         "batched_vs_plain": (value / plain_value) if plain_value else None,
         "roofline": dict(roof, **{
-            "kernel": "k_pairs<P=%d,KMAX=5> (persistent form)" % dom,
+            "kernel": ("k_pearson_lanes (one pair per lane, persistent form)" if dom == -1
+                       else "k_pairs<P=%d,KMAX=5> (persistent form)" % dom),
             "traffic": TRAFFIC_BYTES_PER_LAUNCH if args.config == "H" else None,
             "traffic_source": TRAFFIC_SOURCE if args.config == "H" else None,
             "whole_job_tflops": whole_tflops,

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -34,6 +35,7 @@ KINC_FOR_EACH_INSTANCE(KINC_DECL)
 const void* kinc_kernel_count_valid();
 const void* kinc_kernel_chunk_max();
 const void* kinc_kernel_prev_greater();
+const void* kinc_kernel_pearson_lanes();
 const void* kinc_kernel_bin_pairs();
 const void* kinc_kernel_zero_i32();
 }
@@ -322,11 +324,13 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    a.np2 = next_pow2(S);
    a.tile_stride = KINC_TILE_STRIDE;
    a.mem_slots = (S + 63) / 64;
-   if ( params->corr_method == KINC_CORR_PEARSON && h->classes[0] != 0 )
+   if ( params->corr_method == KINC_CORR_PEARSON &&
+        (h->classes[0] != 0 || params->clus_method == KINC_CLUS_NONE) &&
+        sizeof(float) * 2 * B * (size_t)S <= ((size_t)4 << 30) )
    {
       /* Pearson's stale reads copy earlier pairs' compact X from here
-       * instead of fetching and compacting their rows again (at most
-       * 32768 x 1024 x 8 bytes = 256 MiB) */
+       * instead of fetching and compacting their rows again (256 MiB for
+       * 32768 pairs of 1024 samples; at most 4 GiB) */
       CREATE_TRY(hipMalloc(&h->d_xbuf, sizeof(float) * 2 * B * (size_t)S));
       CREATE_TRY(hipMalloc(&h->d_prev, sizeof(int32_t) * B));
       a.xbuf = h->d_xbuf;
@@ -413,10 +417,13 @@ static int take_event(kinc_sim* h, hipEvent_t* ev)
    return KINC_OK;
 }
 
-/* A pair-kernel launch, bracketed by events when profiling is on. */
-static int launch_pairs(kinc_sim* h, int P, const void* fn, unsigned grid, void** args, hipStream_t s)
+/* A pair-kernel launch, bracketed by events when profiling is on (P keys the
+ * timing: the slot count, or -1 for the one-pair-per-lane Pearson kernel). */
+static int launch_pairs(kinc_sim* h, int P, const void* fn, unsigned grid, void** args, hipStream_t s,
+                        size_t lds = SIZE_MAX)
 {
-   if ( !h->profiling ) return launch(h, fn, grid, 64, args, h->lds_bytes, s);
+   if ( lds == SIZE_MAX ) lds = h->lds_bytes;
+   if ( !h->profiling ) return launch(h, fn, grid, 64, args, lds, s);
    kinc_sim::Timed t;
    t.P = P;
    int rc = take_event(h, &t.a);
@@ -424,7 +431,7 @@ static int launch_pairs(kinc_sim* h, int P, const void* fn, unsigned grid, void*
    rc = take_event(h, &t.b);
    if ( rc ) return rc;
    HIP_TRY(h, hipEventRecord(t.a, s));
-   rc = launch(h, fn, grid, 64, args, h->lds_bytes, s);
+   rc = launch(h, fn, grid, 64, args, lds, s);
    if ( rc ) return rc;
    HIP_TRY(h, hipEventRecord(t.b, s));
    h->timed.push_back(t);
@@ -544,6 +551,15 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       return KINC_OK;
    }
 
+   if ( h->p.clus_method == KINC_CLUS_NONE && h->p.corr_method == KINC_CORR_PEARSON && h->d_xbuf )
+   {
+      /* no fits, Pearson: one pair per lane over the whole block */
+      a.list = nullptr;
+      a.first = 0;
+      a.launch_count = cnt;
+      void* args[] = {&a};
+      return launch_pairs(h, -1, kinc_kernel_pearson_lanes(), (unsigned)((cnt + 63) / 64), args, s, 0);
+   }
    if ( h->p.clus_method == KINC_CLUS_NONE )
    {
       /* no fits: the slot count does not matter and every pair costs about

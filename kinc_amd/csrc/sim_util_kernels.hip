@@ -1,8 +1,9 @@
 /*
  * sim_util_kernels.hip — block preparation kernels of the pairwise
  * Similarity path: valid-sample counts and compact X, chunk maxima and
- * previous-greater links for the stale-read view, sample-count binning for
- * the persistent form, and a zero fill.
+ * previous-greater links for the stale-read view, Pearson without
+ * clustering with one pair per lane, sample-count binning for the
+ * persistent form, and a zero fill.
  */
 #include "sim_pair_impl.h"
 
@@ -110,6 +111,119 @@ __global__ __launch_bounds__(256) void k_bin_pairs(const int32_t* N, int count, 
    lists[(size_t)b * count + pos] = q;
 }
 
+/*
+ * Pearson without clustering, one pair per lane (pairwise_pearson.cpp:12-52
+ * with K = 1, fetchPair's labels, similarity_serial.cpp:108-141).  Lane l of
+ * workgroup b owns block-relative pair q = first + 64 b + l and walks its
+ * samples in index order, keeping the five moment sums in registers.  The
+ * value Pearson reads at a valid index i is the block workspace entry i:
+ * the pair's own compact X below N, then the chain of earlier pairs with
+ * more valid samples (prev links), all read from the compact-X buffer that
+ * k_count_valid wrote; 0 past the chain, as in a fresh workspace.  Eight
+ * indices' loads are issued before any of them is used.
+ */
+__global__ __launch_bounds__(64) void k_pearson_lanes(SimArgs a)
+{
+   const int lane = lane_id();
+   const int q = a.first + (int)blockIdx.x * WAVE + lane;
+   const bool act = q < a.first + a.launch_count;
+   const int S = a.S;
+   const int MC = a.max_k;
+   int gx = 0, gy = 0;
+   if ( act ) pair_genes(a, q, gx, gy);
+   const float* r1 = a.emx + (size_t)gx * (size_t)S;
+   const float* r2 = a.emx + (size_t)gy * (size_t)S;
+   const float2* xb = (const float2*)a.xbuf;
+   int8_t* lab_out = (act && a.labels_out) ? a.labels_out + (size_t)q * (size_t)S : nullptr;
+   const float me = a.min_expr;
+
+   /* workspace entries [.., hi) come from pair d; d = -1: zeros */
+   int d = act ? q : -1;
+   int hi = act ? a.blockN[q] : S;
+   int n = 0;
+   float sx = 0.0f, sy = 0.0f, sxx = 0.0f, syy = 0.0f, sxy = 0.0f;
+   double dx = 0.0, dy = 0.0, dxx = 0.0, dyy = 0.0, dxy = 0.0;
+
+   constexpr int U = 8;
+   for ( int i0 = 0; i0 < S; i0 += U )
+   {
+      int src[U];
+#pragma unroll
+      for ( int j = 0; j < U; ++j )
+      {
+         const int i = i0 + j;
+         while ( d >= 0 && i >= hi && i < S )
+         {
+            d = a.prev[d];
+            hi = d >= 0 ? a.blockN[d] : S;
+         }
+         src[j] = d;
+      }
+      float g1[U], g2[U];
+      float2 w[U];
+#pragma unroll
+      for ( int j = 0; j < U; ++j )
+      {
+         const int i = i0 + j;
+         const bool in = act && i < S;
+         g1[j] = in ? r1[i] : 0.0f;
+         g2[j] = in ? r2[i] : 0.0f;
+         w[j] = (in && src[j] >= 0) ? xb[(size_t)src[j] * (size_t)S + i] : make_float2(0.0f, 0.0f);
+      }
+#pragma unroll
+      for ( int j = 0; j < U; ++j )
+      {
+         const int i = i0 + j;
+         if ( !act || i >= S ) continue;
+         const bool nan = isnan(g1[j]) || isnan(g2[j]);
+         const bool low = !nan && (g1[j] < me || g2[j] < me);
+         if ( lab_out ) lab_out[i] = nan ? (int8_t)-9 : (low ? (int8_t)-6 : (int8_t)0);
+         if ( nan || low ) continue;
+         const float x = w[j].x;
+         const float y = w[j].y;
+         ++n;
+         if ( !a.pearson_fp64 )
+         {
+            sx = sx + x;
+            sy = sy + y;
+            sxx = sxx + x * x;
+            syy = syy + y * y;
+            sxy = sxy + x * y;
+         }
+         else
+         {
+            const double xd = x, yd = y;
+            dx = dx + xd;
+            dy = dy + yd;
+            dxx = dxx + xd * xd;
+            dyy = dyy + yd * yd;
+            dxy = dxy + xd * yd;
+         }
+      }
+   }
+   if ( !act ) return;
+
+   float r = NAN;
+   if ( n >= a.min_samples && !a.pearson_fp64 )
+   {
+      const float fn = (float)n;
+      r = (fn * sxy - sx * sy) / sqrtf((fn * sxx - sx * sx) * (fn * syy - sy * sy));
+   }
+   else if ( n >= a.min_samples )
+   {
+      const double dn = (double)n;
+      r = (float)((dn * dxy - dx * dy) / sqrt((dn * dxx - dx * dx) * (dn * dyy - dy * dy)));
+   }
+   a.K_out[q] = 1;
+   float* co = a.corr_out + (size_t)q * MC;
+   co[0] = r;
+   for ( int k = 1; k < MC; ++k ) co[k] = NAN;
+   if ( a.iters_out )
+   {
+      for ( int k = 0; k < MC; ++k ) a.iters_out[(size_t)q * MC + k] = 0;
+   }
+}
+
 /* Zero a small int array (used instead of a memset node). */
 __global__ void k_zero_i32(int32_t* p, int n)
 {
@@ -122,6 +236,7 @@ extern "C" {
 const void* kinc_kernel_count_valid() { return (const void*)&k_count_valid; }
 const void* kinc_kernel_chunk_max() { return (const void*)&k_chunk_max; }
 const void* kinc_kernel_prev_greater() { return (const void*)&k_prev_greater; }
+const void* kinc_kernel_pearson_lanes() { return (const void*)&k_pearson_lanes; }
 const void* kinc_kernel_bin_pairs() { return (const void*)&k_bin_pairs; }
 const void* kinc_kernel_zero_i32() { return (const void*)&k_zero_i32; }
 }

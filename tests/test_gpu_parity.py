@@ -70,6 +70,14 @@ CASES = [
           pearson_fp64=True), 0, 200),
     ("gmm_pearson_fp64_S1100", dict(kind="modules", G=10, S=1100, seed=40, nan_rate=0.05),
      dict(clusmethod="gmm", corrmethod="pearson", crit="ICL", pearson_fp64=True), 0, 45),
+    # Pearson without clustering (one pair per lane in the persistent form):
+    # expression floor, above the register range, a ragged launch at an offset
+    ("pearson_none_minexpr", dict(kind="latent", G=90, S=150, seed=41, nan_rate=0.05),
+     dict(clusmethod="none", corrmethod="pearson", minexpr=7.0), 0, 4005),
+    ("pearson_none_S1300", dict(kind="latent", G=40, S=1300, seed=42, nan_rate=0.10),
+     dict(clusmethod="none", corrmethod="pearson"), 0, 780),
+    ("pearson_none_ragged", dict(kind="latent", G=50, S=77, seed=43, nan_rate=0.30),
+     dict(clusmethod="none", corrmethod="pearson", minsamp=20), 100, 1000),
 ]
 
 
