@@ -519,7 +519,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       int32_t* cm = h->d_chunkMax;
       void* args[] = {&Np, (void*)&cnt, &cm};
       const int chunks = (cnt + 63) / 64;
-      int rc = launch(h, kinc_kernel_chunk_max(), (unsigned)((chunks + 255) / 256), 256, args, 0, s);
+      int rc = launch(h, kinc_kernel_chunk_max(), (unsigned)((chunks + 3) / 4), 256, args, 0, s);
       if ( rc ) return rc;
       if ( h->d_prev )
       {

@@ -73,17 +73,19 @@ __global__ __launch_bounds__(256) void k_prev_greater(const int32_t* N, const in
    if ( lane_id() == 0 ) prev[q] = d;
 }
 
-/* max of N over 64-pair chunks */
+/* max of N over 64-pair chunks (one wave per chunk, 4 per group) */
 __global__ __launch_bounds__(256) void k_chunk_max(const int32_t* N, int count, int32_t* cmax)
 {
-   const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-   const int lo = c * WAVE;
-   if ( lo >= count ) return;
-   int hi = lo + WAVE;
-   hi = hi < count ? hi : count;
-   int m = -1;
-   for ( int i = lo; i < hi; ++i ) m = N[i] > m ? N[i] : m;
-   cmax[c] = m;
+   const int c = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+   if ( c * WAVE >= count ) return;
+   const int i = c * WAVE + lane_id();
+   int m = i < count ? N[i] : -1;
+   for ( int off = WAVE / 2; off > 0; off >>= 1 )
+   {
+      const int o = __shfl_xor(m, off);
+      m = o > m ? o : m;
+   }
+   if ( lane_id() == 0 ) cmax[c] = m;
 }
 
 /*
