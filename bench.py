@@ -360,6 +360,8 @@ def main():
                     help="time the CPU oracle on one thread only")
     ap.add_argument("--config", default="H", choices=["A", "B", "H", "C"],
                     help="workload (H is the headline; the others are reported for comparison)")
+    ap.add_argument("--clusmethod", choices=["none", "gmm", "kmeans"], default=None,
+                    help="override the workload's clustering method (config C runs both)")
     ap.add_argument("--pearson-fp64", action="store_true",
                     help="Pearson sums and quotient in double (the pearson_fp64 option)")
     ap.add_argument("--no-serve", action="store_true")
@@ -402,6 +404,8 @@ def main():
     flags = dict(cfg["params"])
     if args.pearson_fp64:
         flags["pearson_fp64"] = True
+    if args.clusmethod:
+        flags["clusmethod"] = args.clusmethod
     t = time.time()
     x = gemgen.make(args.config)
     G, S = x.shape
@@ -510,7 +514,8 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": WORKLOADS[args.config],
+            "workload": WORKLOADS[args.config] + (
+                "; clustering overridden: %s" % args.clusmethod if args.clusmethod else ""),
             "pairs_per_step": BLOCK,
             "step": "one 32768-pair work block",
             "form": "persistent",
