@@ -410,6 +410,25 @@ __device__ __forceinline__ float chain_add(const float* tile, int stride, int nc
    return acc;
 }
 
+/* chain_add for a length that is a multiple of 4 (at most the tile width). */
+__device__ __forceinline__ float chain_add4(const float* tile, int stride, int nch, int len4, float acc)
+{
+   const int lane = lane_id();
+   if ( lane < nch || lane < KINC_MIN_ACTIVE_LANES )
+   {
+      const float4* r4 = (const float4*)(tile + (lane < nch ? lane : nch - 1) * stride);
+      for ( int j = 0; j < (len4 >> 2); ++j )
+      {
+         float4 v = r4[j];
+         acc = acc + v.x;
+         acc = acc + v.y;
+         acc = acc + v.z;
+         acc = acc + v.w;
+      }
+   }
+   return acc;
+}
+
 /* ---------------------------------------------------------- point stores */
 
 /*
@@ -994,8 +1013,16 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
             if ( m < nslots ) st.Y(m) = st.YN(m);
          }
 
-         /* means: chain 2k / 2k+1 over members in index order */
+         /* means: chain 2k / 2k+1 over the members of cluster k in index
+          * order.  Each slot's members of k are packed to the front of rows
+          * 2k and 2k+1 and the rows are zero-filled to a common length (a
+          * multiple of 4), so a chain adds only its members plus zeros; the
+          * running sums start at +0 and never become -0, so the zeros leave
+          * them unchanged. */
          float acc = 0.0f;
+         int nkc[K];
+#pragma unroll
+         for ( int k = 0; k < K; ++k ) nkc[k] = 0;
 #pragma unroll
          for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
          {
@@ -1003,33 +1030,41 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
             {
                const float xa = st.X0(m);
                const float xb = st.X1(m);
-               const int ym = st.Y(m);
+               const int ym = (lane + m * WAVE < N) ? (int)st.Y(m) : -1;
+               int ck[K];
+               int rank = 0, maxc = 0;
 #pragma unroll
                for ( int k = 0; k < K; ++k )
                {
-                  const bool mem = ym == k;
-                  tile[(2 * k) * STR + lane] = mem ? xa : 0.0f;
-                  tile[(2 * k + 1) * STR + lane] = mem ? xb : 0.0f;
+                  const uint64_t bm = ballot(ym == k);
+                  ck[k] = popc(bm);
+                  nkc[k] += ck[k];
+                  maxc = ck[k] > maxc ? ck[k] : maxc;
+                  rank = (ym == k) ? popc(bm & lanes_below()) : rank;
+               }
+               const int len = (maxc + 3) & ~3;
+               if ( ym >= 0 )
+               {
+                  tile[(2 * ym) * STR + rank] = xa;
+                  tile[(2 * ym + 1) * STR + rank] = xb;
+               }
+#pragma unroll
+               for ( int k = 0; k < K; ++k )
+               {
+                  if ( lane >= ck[k] && lane < len )
+                  {
+                     tile[(2 * k) * STR + lane] = 0.0f;
+                     tile[(2 * k + 1) * STR + lane] = 0.0f;
+                  }
                }
                wave_sync(L.lgbuf != nullptr);
-               int len = N - m * WAVE;
-               len = len < TILE ? len : TILE;
-               acc = chain_add(tile, STR, 2 * K, len, acc);
+               acc = chain_add4(tile, STR, 2 * K, len, acc);
                wave_sync(L.lgbuf != nullptr);
             }
          }
          int nk = 0;
 #pragma unroll
-         for ( int k = 0; k < K; ++k )
-         {
-            int cnt = 0;
-#pragma unroll
-            for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
-            {
-               if ( m < nslots ) cnt += popc(ballot(lane + m * WAVE < N && st.Y(m) == k));
-            }
-            nk = (lane == k) ? cnt : nk;
-         }
+         for ( int k = 0; k < K; ++k ) nk = (lane == k) ? nkc[k] : nk;
          const float sx = __shfl(acc, 2 * lane);
          const float sy = __shfl(acc, 2 * lane + 1);
          const float c = 1.0f / (float)nk;
