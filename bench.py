@@ -50,6 +50,11 @@ WORKLOADS = {
 # WRITE_SIZE passes (scaled by a copy kernel of known size) recorded in:
 TRAFFIC_SOURCE = "profiles/k_pairs_gmm_spearman_20000x500_counters.md"
 TRAFFIC_BYTES_PER_LAUNCH = 634.4e6
+# VALU wave64 instructions per second of the same kernel (SQ_INSTS_VALU per
+# dispatch / duration, same record) against the issue ceiling: 1024 SIMDs x
+# 2.4 GHz / 4 cycles per wave64 instruction
+VALU_ISSUE_RATE = 7.458e11
+VALU_ISSUE_CEILING = 1024 * 2.4e9 / 4
 BLOCK = 32768
 
 
@@ -529,6 +534,9 @@ def main():
                        else "k_pairs<P=%d,KMAX=5> (persistent form)" % dom),
             "traffic": TRAFFIC_BYTES_PER_LAUNCH if args.config == "H" else None,
             "traffic_source": TRAFFIC_SOURCE if args.config == "H" else None,
+            "valu_issue": ({"insts_per_s": VALU_ISSUE_RATE, "ceiling": VALU_ISSUE_CEILING,
+                            "frac": VALU_ISSUE_RATE / VALU_ISSUE_CEILING,
+                            "source": TRAFFIC_SOURCE} if args.config == "H" else None),
             "whole_job_tflops": whole_tflops,
         }),
         "cpu_baseline": cpu,
