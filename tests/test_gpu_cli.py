@@ -76,3 +76,17 @@ def test_cli_pearson_several_blocks(tmp_path):
         assert formats.read_file(a)[1] == formats.read_file(b)[1]
     hdr, rec = formats.read_pairwise(cmx)
     assert hdr["pairs"] > 1000
+
+
+def test_cli_pearson_fp64_option(tmp_path):
+    # --pearson-fp64 true reaches the engine: the files match the writer fed
+    # with the oracle's FP64-Pearson results, and differ from the default's
+    x = gemgen.latent(150, 120, seed=8, nan_rate=0.1)
+    flags = dict(clusmethod="none", corrmethod="pearson", minsamp=20, mincorr=0.1)
+    ccm, cmx, meta = run_cli(tmp_path, x, dict(flags, **{"pearson-fp64": True}), "persistent")
+    occm, ocmx = oracle_files(tmp_path, x, dict(flags, pearson_fp64=True), meta)
+    assert formats.read_file(cmx)[1] == formats.read_file(ocmx)[1]
+    d32 = tmp_path / "d32"
+    d32.mkdir()
+    c32, x32, _ = run_cli(d32, x, flags, "persistent")
+    assert formats.read_file(x32)[1] != formats.read_file(cmx)[1]
