@@ -20,11 +20,20 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 #include "../../include/kinc_formats.h"
 #include "../../include/kinc_sim.h"
+
+/* Pairs of a process() call per encoding thread, at least (thread start-up
+ * costs about as much as encoding this many single-cluster pairs). */
+#ifndef KINC_WRITER_PAIRS_PER_THREAD
+#define KINC_WRITER_PAIRS_PER_THREAD 2048
+#endif
 
 static_assert(sizeof(float) == 4, "float32");
 
@@ -208,6 +217,98 @@ struct PairwiseFile
 
 } // namespace
 
+namespace {
+
+/* The records of a run of pairs, encoded into memory. */
+struct Encoded
+{
+   std::string ccm, cmx;
+   int64_t ccm_recs = 0, ccm_pairs = 0, cmx_recs = 0, cmx_pairs = 0;
+   int64_t ccm_first = -1, ccm_last = -1, cmx_first = -1, cmx_last = -1;
+   int64_t bad = -1;        /* first pair that cannot be written */
+   std::string why;
+   /* empty again, keeping the buffers' capacity */
+   void reset()
+   {
+      ccm.clear();
+      cmx.clear();
+      ccm_recs = ccm_pairs = cmx_recs = cmx_pairs = 0;
+      ccm_first = ccm_last = cmx_first = cmx_last = -1;
+      bad = -1;
+      why.clear();
+   }
+};
+
+/* A fixed set of worker threads that run the tasks of one call at a time;
+ * the caller runs task 0 itself and waits for the rest. */
+class WorkerPool
+{
+public:
+   explicit WorkerPool(int workers)
+   {
+      for ( int t = 1; t <= workers; ++t ) th_.emplace_back([this, t] { loop(t); });
+   }
+   ~WorkerPool()
+   {
+      {
+         std::lock_guard<std::mutex> g(m_);
+         stop_ = true;
+      }
+      cv_.notify_all();
+      for ( auto& t : th_ ) t.join();
+   }
+   int size() const { return (int)th_.size() + 1; }
+   /* f(t) for t in [0, tasks), tasks <= size() */
+   void run(int tasks, const std::function<void(int)>& f)
+   {
+      {
+         std::lock_guard<std::mutex> g(m_);
+         job_ = &f;
+         tasks_ = tasks;
+         pending_ = tasks - 1;
+         ++gen_;
+      }
+      cv_.notify_all();
+      f(0);
+      std::unique_lock<std::mutex> g(m_);
+      done_.wait(g, [this] { return pending_ == 0; });
+      job_ = nullptr;
+   }
+
+private:
+   void loop(int t)
+   {
+      uint64_t seen = 0;
+      for ( ;; )
+      {
+         const std::function<void(int)>* job = nullptr;
+         {
+            std::unique_lock<std::mutex> g(m_);
+            cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+            if ( stop_ ) return;
+            seen = gen_;
+            if ( t < tasks_ ) job = job_;
+         }
+         if ( job )
+         {
+            (*job)(t);
+            std::lock_guard<std::mutex> g(m_);
+            if ( --pending_ == 0 ) done_.notify_one();
+         }
+      }
+   }
+   std::vector<std::thread> th_;
+   std::mutex m_;
+   std::condition_variable cv_, done_;
+   const std::function<void(int)>* job_ = nullptr;
+   int tasks_ = 0;
+   int pending_ = 0;
+   uint64_t gen_ = 0;
+   bool stop_ = false;
+};
+
+} // namespace
+
 struct kinc_similarity_writer
 {
    PairwiseFile ccm;
@@ -216,6 +317,9 @@ struct kinc_similarity_writer
    int32_t max_clusters = 0;
    float min_corr = 0.5f;
    float max_corr = 1.0f;
+   /* encoding threads and their buffers, kept across process() calls */
+   std::unique_ptr<WorkerPool> pool;
+   std::vector<Encoded> parts;
 };
 
 extern "C" const char* kinc_formats_last_error(void)
@@ -379,16 +483,6 @@ const NibbleTable& nibble_table()
    return table;
 }
 
-/* The records of a run of pairs, encoded into memory. */
-struct Encoded
-{
-   std::string ccm, cmx;
-   int64_t ccm_recs = 0, ccm_pairs = 0, cmx_recs = 0, cmx_pairs = 0;
-   int64_t ccm_first = -1, ccm_last = -1, cmx_first = -1, cmx_last = -1;
-   int64_t bad = -1;        /* first pair that cannot be written */
-   std::string why;
-};
-
 inline void put_header(std::string& o, int32_t x, int32_t y, int c)
 {
    char h[9];
@@ -428,6 +522,45 @@ extern "C" int kinc_similarity_writer_process(kinc_similarity_writer* w, int64_t
       {
          const int k_sel = K[q];
          const float* r = corr + q * corr_stride;
+         if ( k_sel == 1 )
+         {
+            /* a run of single-cluster pairs: no CCM record, at most one CMX
+             * record each, written straight into the buffer */
+            size_t at = e.cmx.size();
+            e.cmx.resize(at + (size_t)(q1 - q) * 13);
+            char* o = &e.cmx[0];
+            int64_t first = e.cmx_first, last = e.cmx_last, recs = 0;
+            for ( ; q < q1 && K[q] == 1; ++q )
+            {
+               /* branch-free: the record is always written and kept by
+                * advancing past it (a NaN fails both comparisons) */
+               const float c = corr[q * corr_stride];
+               const float a = std::fabs(c);
+               const bool keep = (lo <= a) & (a <= hi);
+               const uint32_t ux = (uint32_t)x, uy = (uint32_t)y;
+               memcpy(o + at, &ux, 4);      /* little-endian host */
+               memcpy(o + at + 4, &uy, 4);
+               o[at + 8] = 0;
+               memcpy(o + at + 9, &c, 4);
+               at += keep ? 13 : 0;
+               const int64_t indent = ((int64_t)x * (x - 1) / 2 + y) * KINC_MAX_CLUSTER_SIZE;
+               first = (keep & (first < 0)) ? indent : first;
+               last = keep ? indent : last;
+               recs += keep;
+               if ( ++y >= x )
+               {
+                  y = 0;
+                  ++x;
+               }
+            }
+            e.cmx_first = first;
+            e.cmx_last = last;
+            e.cmx_recs += recs;
+            e.cmx_pairs += recs;
+            e.cmx.resize(at);
+            --q;
+            continue;
+         }
          if ( k_sel > MC )
          {
             e.bad = q;
@@ -493,22 +626,24 @@ extern "C" int kinc_similarity_writer_process(kinc_similarity_writer* w, int64_t
       }
    };
 
-   int T = (int)std::thread::hardware_concurrency();
-   T = std::max(1, std::min(T, 16));
-   T = (int)std::min<int64_t>(T, std::max<int64_t>(1, count / 2048));
-   std::vector<Encoded> parts((size_t)T);
+   if ( !w->pool )
+   {
+      int hw = (int)std::thread::hardware_concurrency();
+      w->pool.reset(new WorkerPool(std::max(1, std::min(hw, 16)) - 1));
+      w->parts.resize((size_t)w->pool->size());
+   }
+   int T = w->pool->size();
+   T = (int)std::min<int64_t>(T, std::max<int64_t>(1, count / KINC_WRITER_PAIRS_PER_THREAD));
+   std::vector<Encoded>& parts = w->parts;
+   for ( int t = 0; t < T; ++t ) parts[(size_t)t].reset();
    if ( T == 1 ) encode(0, count, parts[0]);
    else
    {
-      std::vector<std::thread> pool;
-      for ( int t = 0; t < T; ++t )
-      {
-         pool.emplace_back(encode, count * t / T, count * (t + 1) / T, std::ref(parts[(size_t)t]));
-      }
-      for ( auto& th : pool ) th.join();
+      w->pool->run(T, [&](int t) { encode(count * t / T, count * (t + 1) / T, parts[(size_t)t]); });
    }
-   for ( const Encoded& e : parts )
+   for ( int t = 0; t < T; ++t )
    {
+      const Encoded& e = parts[(size_t)t];
       int rc = w->ccm.append(e.ccm, e.ccm_recs, e.ccm_first, e.ccm_last);
       if ( rc == KINC_OK ) rc = w->cmx.append(e.cmx, e.cmx_recs, e.cmx_first, e.cmx_last);
       if ( rc ) return rc;
