@@ -311,18 +311,22 @@ int main(int argc, char** argv)
       return 1;
    }
 
+   /* without clustering every pair has K = 1 and the writer needs no labels */
+   const bool want_labels = p.clus_method != KINC_CLUS_NONE;
    auto t0 = std::chrono::steady_clock::now();
    auto submit = [&](int64_t b) -> bool {
       const int s = (int)(b & 1);
       const int64_t start = b * B;
       const int64_t count = std::min(B, total - start);
-      if ( kinc_sim_execute_device(h, start, count, dev[s].K, dev[s].labels, dev[s].corr, nullptr, stream) != KINC_OK )
+      if ( kinc_sim_execute_device(h, start, count, dev[s].K, want_labels ? dev[s].labels : nullptr,
+                                   dev[s].corr, nullptr, stream) != KINC_OK )
       {
          fprintf(stderr, "%s\n", kinc_sim_last_error(h));
          return false;
       }
       return hipMemcpyAsync(host[s].K, dev[s].K, count, hipMemcpyDeviceToHost, stream) == hipSuccess
-          && hipMemcpyAsync(host[s].labels, dev[s].labels, count * S, hipMemcpyDeviceToHost, stream) == hipSuccess
+          && (!want_labels
+              || hipMemcpyAsync(host[s].labels, dev[s].labels, count * S, hipMemcpyDeviceToHost, stream) == hipSuccess)
           && hipMemcpyAsync(host[s].corr, dev[s].corr, sizeof(float) * count * MC, hipMemcpyDeviceToHost, stream) == hipSuccess
           && hipEventRecord(done[s], stream) == hipSuccess;
    };
@@ -343,7 +347,8 @@ int main(int argc, char** argv)
       }
       const int64_t start = b * B;
       const int64_t count = std::min(B, total - start);
-      if ( kinc_similarity_writer_process(w, start, count, host[s].K, host[s].labels, host[s].corr, MC) != KINC_OK )
+      if ( kinc_similarity_writer_process(w, start, count, host[s].K, want_labels ? host[s].labels : nullptr,
+                                          host[s].corr, MC) != KINC_OK )
       {
          fprintf(stderr, "%s\n", kinc_formats_last_error());
          rc = 1;
