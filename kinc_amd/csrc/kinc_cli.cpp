@@ -9,8 +9,9 @@
  *        [--minexpr F] [--minsamp N] [--minclus N] [--maxclus N]
  *        [--crit BIC|ICL] [--preout true|false] [--postout true|false]
  *        [--mincorr F] [--maxcorr F] [--ksize N]
- * plus this build's --device N, --form plain|persistent and --pearson-fp64
- * BOOL (Pearson sums and quotient in double); and the
+ * plus three flags the reference does not have: --device N, --form
+ * plain|persistent and --pearson-fp64 BOOL (Pearson sums and quotient in
+ * double); and the
  * expression-matrix converters around it (importexpressionmatrix_input.cpp,
  * exportexpressionmatrix_input.cpp):
  *   kinc run import-emx --input A.txt --output A.emx [--nan TOKEN]
