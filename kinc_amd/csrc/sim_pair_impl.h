@@ -559,8 +559,60 @@ __device__ __forceinline__ float lane_pick(const float (&v)[K])
  * LPX(m) = log p(x) (pairwise_gmm.cpp:54-77,143-187).  Lanes past the
  * valid points compute harmless values that are never read. */
 template <class ST, int K>
-__device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm)
+__device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, float* vmax = nullptr)
 {
+   if constexpr ( ST::PMAX == 0 )
+   {
+      /* memory store: slot by slot, so each responsibility is written once
+       * (the same arithmetic as below, in the same order per element); with
+       * vmax, also each lane's maximum of every component's log
+       * responsibility over its valid points */
+      if ( vmax )
+      {
+#pragma unroll
+         for ( int k = 0; k < K; ++k ) vmax[k] = -INFINITY;
+      }
+      float4 A[K], B[K];
+#pragma unroll
+      for ( int k = 0; k < K; ++k )
+      {
+         A[k] = *(const float4*)(prm + 8 * k);
+         B[k] = *(const float4*)(prm + 8 * k + 4);
+      }
+      for ( int m = 0; m < nslots; ++m )
+      {
+         const float x0 = st.X0(m);
+         const float x1 = st.X1(m);
+         float lgm[K];
+#pragma unroll
+         for ( int k = 0; k < K; ++k )
+         {
+            float xm0 = x0 - A[k].x;
+            float xm1 = x1 - A[k].y;
+            float b0 = A[k].z * xm0 + A[k].w * xm1;
+            float b1 = B[k].x * xm0 + B[k].y * xm1;
+            float dot = xm0 * b0 + xm1 * b1;
+            lgm[k] = B[k].z - 0.5f * dot;
+         }
+         float mx = -INFINITY;
+#pragma unroll
+         for ( int k = 0; k < K; ++k ) mx = fmaxf(mx, B[k].w + lgm[k]);
+         float sum = 0.0f;
+#pragma unroll
+         for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf((B[k].w + lgm[k]) - mx);
+         const float lp = mx + kinc_logf(sum);
+         st.LPX(m) = lp;
+         const bool valid = lane_id() + m * WAVE < st.n;
+#pragma unroll
+         for ( int k = 0; k < K; ++k )
+         {
+            const float l = lgm[k] + (-lp);
+            st.LG(k, m) = l;
+            if ( vmax ) vmax[k] = valid ? fmaxf(vmax[k], l) : vmax[k];
+         }
+      }
+      return;
+   }
    float logpi[K];
 #pragma unroll
    for ( int k = 0; k < K; ++k )
@@ -659,7 +711,8 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
 
    for ( int t = 0; t < 100; ++t )
    {
-      gmm_estep<ST, K>(nslots, st, prm);
+      float vmax[K];
+      gmm_estep<ST, K>(nslots, st, prm, vmax);
       e_steps++;
 
       /* max over the points of each component's log responsibility; only
@@ -670,6 +723,11 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
 #pragma unroll
       for ( int k = 0; k < K; ++k )
       {
+         if constexpr ( ST::PMAX == 0 )
+         {
+            gmax[k] = wave_max(vmax[k]);   /* per-lane maxima from the E step */
+            continue;
+         }
          float v = -INFINITY;
 #pragma unroll
          for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
