@@ -78,6 +78,16 @@ CASES = [
      dict(clusmethod="none", corrmethod="pearson"), 0, 780),
     ("pearson_none_ragged", dict(kind="latent", G=50, S=77, seed=43, nan_rate=0.30),
      dict(clusmethod="none", corrmethod="pearson", minsamp=20), 100, 1000),
+    # smallest shapes: one pair; fewer samples than a slot
+    ("gmm_spearman_one_pair", dict(kind="modules", G=2, S=90, seed=44, nan_rate=0.05),
+     dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True,
+          minsamp=10), 0, 1),
+    ("gmm_spearman_S24", dict(kind="modules", G=30, S=24, seed=45, nan_rate=0.05),
+     dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", minsamp=10), 0, 435),
+    ("pearson_none_S24", dict(kind="latent", G=30, S=24, seed=46, nan_rate=0.10),
+     dict(clusmethod="none", corrmethod="pearson", minsamp=10), 0, 435),
+    ("kmeans_pearson_maxclus8", dict(kind="modules", G=20, S=300, seed=47, nan_rate=0.05),
+     dict(clusmethod="kmeans", corrmethod="pearson", crit="BIC", maxclus=8), 0, 190),
 ]
 
 
