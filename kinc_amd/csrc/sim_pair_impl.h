@@ -558,15 +558,29 @@ __device__ __forceinline__ float lane_pick(const float (&v)[K])
 /* E step for the slots in use: LG(k,m) = log N(x|k) - log p(x),
  * LPX(m) = log p(x) (pairwise_gmm.cpp:54-77,143-187).  Lanes past the
  * valid points compute harmless values that are never read. */
+/* log N(x | component) (pairwise_gmm.cpp:54-77) from the component's
+ * parameters A = (mu0, mu1, a00, a01), B = (a10, a11, normalizer, logpi). */
+__device__ __forceinline__ float gmm_logpdf(float x0, float x1, const float4& A, const float4& B)
+{
+   float xm0 = x0 - A.x;
+   float xm1 = x1 - A.y;
+   float b0 = A.z * xm0 + A.w * xm1;
+   float b1 = B.x * xm0 + B.y * xm1;
+   float dot = xm0 * b0 + xm1 * b1;
+   return B.z - 0.5f * dot;
+}
+
 template <class ST, int K>
-__device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, float* vmax = nullptr)
+__device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, float* vmax = nullptr,
+                                          bool store_lg = true)
 {
    if constexpr ( ST::PMAX == 0 )
    {
       /* memory store: slot by slot, so each responsibility is written once
        * (the same arithmetic as below, in the same order per element); with
        * vmax, also each lane's maximum of every component's log
-       * responsibility over its valid points */
+       * responsibility over its valid points; without store_lg only log p(x)
+       * is stored and the next pass recomputes the responsibilities */
       if ( vmax )
       {
 #pragma unroll
@@ -585,15 +599,7 @@ __device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, 
          const float x1 = st.X1(m);
          float lgm[K];
 #pragma unroll
-         for ( int k = 0; k < K; ++k )
-         {
-            float xm0 = x0 - A[k].x;
-            float xm1 = x1 - A[k].y;
-            float b0 = A[k].z * xm0 + A[k].w * xm1;
-            float b1 = B[k].x * xm0 + B[k].y * xm1;
-            float dot = xm0 * b0 + xm1 * b1;
-            lgm[k] = B[k].z - 0.5f * dot;
-         }
+         for ( int k = 0; k < K; ++k ) lgm[k] = gmm_logpdf(x0, x1, A[k], B[k]);
          float mx = -INFINITY;
 #pragma unroll
          for ( int k = 0; k < K; ++k ) mx = fmaxf(mx, B[k].w + lgm[k]);
@@ -607,7 +613,7 @@ __device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, 
          for ( int k = 0; k < K; ++k )
          {
             const float l = lgm[k] + (-lp);
-            st.LG(k, m) = l;
+            if ( store_lg ) st.LG(k, m) = l;
             if ( vmax ) vmax[k] = valid ? fmaxf(vmax[k], l) : vmax[k];
          }
       }
@@ -712,7 +718,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
    for ( int t = 0; t < 100; ++t )
    {
       float vmax[K];
-      gmm_estep<ST, K>(nslots, st, prm, vmax);
+      gmm_estep<ST, K>(nslots, st, prm, vmax, ST::PMAX != 0);
       e_steps++;
 
       /* max over the points of each component's log responsibility; only
@@ -745,23 +751,41 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          gmax[k] = wave_max(v);
       }
 
-      /* pass A: chain 0 = logL, 1+k = Gamma_k sum, 1+K+2k / 2+K+2k = mu_k sums */
+      /* pass A: chain 0 = logL, 1+k = Gamma_k sum, 1+K+2k / 2+K+2k = mu_k sums.
+       * The memory store recomputes the log responsibilities from X and the
+       * parameters of this E step (identical arithmetic) instead of reading
+       * them back. */
       float acc = 0.0f;
+      float4 PA[ST::PMAX == 0 ? K : 1], PB[ST::PMAX == 0 ? K : 1];
+      if constexpr ( ST::PMAX == 0 )
+      {
+#pragma unroll
+         for ( int k = 0; k < K; ++k )
+         {
+            PA[k] = *(const float4*)(prm + 8 * k);
+            PB[k] = *(const float4*)(prm + 8 * k + 4);
+         }
+      }
 #pragma unroll
       for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
       {
          if ( m < nslots )
          {
-            tile[lane] = st.LPX(m);
+            const float lp = st.LPX(m);
+            tile[lane] = lp;
             const float xa = st.X0(m);
             const float xb = st.X1(m);
 #pragma unroll
             for ( int k = 0; k < K; ++k )
             {
-               const float l = st.LG(k, m);
+               float l;
+               if constexpr ( ST::PMAX == 0 ) l = gmm_logpdf(xa, xb, PA[k], PB[k]) + (-lp);
+               else l = st.LG(k, m);
                tile[(1 + k) * STR + lane] = kinc_expf(l - gmax[k]);
                const float gm = kinc_expf(l);
-               st.LG(k, m) = gm;
+               /* the memory store keeps gamma only where the labels of a
+                * 100-iteration fit will read it; pass B recomputes it */
+               if ( ST::PMAX != 0 || t == 99 ) st.LG(k, m) = gm;
                tile[(1 + K + 2 * k) * STR + lane] = gm * xa;
                tile[(2 + K + 2 * k) * STR + lane] = gm * xb;
             }
@@ -832,12 +856,17 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          {
             const float xa = st.X0(m);
             const float xb = st.X1(m);
+            float lp = 0.0f;
+            if constexpr ( ST::PMAX == 0 ) lp = st.LPX(m);
 #pragma unroll
             for ( int k = 0; k < K; ++k )
             {
                const float xm0 = xa - muk0[k];
                const float xm1 = xb - muk1[k];
-               const float gm = st.LG(k, m);
+               /* gamma as pass A computed it, from this E step's parameters */
+               float gm;
+               if constexpr ( ST::PMAX == 0 ) gm = kinc_expf(gmm_logpdf(xa, xb, PA[k], PB[k]) + (-lp));
+               else gm = st.LG(k, m);
                tile[(3 * k) * STR + lane] = gm * (xm0 * xm0);
                tile[(3 * k + 1) * STR + lane] = gm * (xm0 * xm1);
                tile[(3 * k + 2) * STR + lane] = gm * (xm1 * xm1);
