@@ -178,3 +178,34 @@ def test_writer_stops_at_the_first_bad_pair(tmp_path):
     want_ccm, want_cmx = _model_process(0, K[:bad], labels[:bad], corr[:bad], S,
                                         np.float32(0.2), 1.0)
     _check_files(ccm, cmx, want_ccm, want_cmx, S)
+
+
+def test_writer_single_cluster_runs_over_many_calls(tmp_path):
+    # long runs of one-cluster pairs (the writer's branch-free path) between
+    # multi-cluster and empty pairs, split over calls of very different sizes
+    # (one encoding thread, then the pool, then one again), with
+    # correlations on the |r| bounds, NaN and zero
+    rng = np.random.default_rng(21)
+    G, S, MC = 180, 9, 5
+    n = G * (G - 1) // 2
+    K, labels, corr = _random_block(rng, n, S, MC)
+    labels = np.where(labels >= MC, 0, labels).astype(np.int8)
+    runs = rng.random(n) < 0.8
+    K[runs] = 1
+    lo, hi = np.float32(0.3), np.float32(0.9)
+    edge = rng.choice(np.array([0.3, -0.3, 0.9, -0.9, 0.0, np.nan], np.float32), n)
+    pick = rng.random(n) < 0.05
+    corr[pick, 0] = edge[pick]
+    ccm, cmx = str(tmp_path / "r.ccm"), str(tmp_path / "r.cmx")
+    w = formats.SimilarityWriter(ccm, cmx, G, S, MC, min_corr=float(lo), max_corr=float(hi))
+    sizes = [1, 7, 2048, 9000, 3, 4096, 130, 20000]
+    s = 0
+    for sz in sizes:
+        e = min(n, s + sz)
+        w.process(s, K[s:e], labels[s:e], corr[s:e])
+        s = e
+    w.process(s, K[s:], labels[s:], corr[s:])
+    w.close()
+    want_ccm, want_cmx = _model_process(0, K, labels, corr, S, lo, hi)
+    _check_files(ccm, cmx, want_ccm, want_cmx, S)
+    assert len(want_cmx) > 5000
