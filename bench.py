@@ -379,6 +379,8 @@ def main():
                     help="time the CPU oracle on one thread only")
     ap.add_argument("--config", default="H", choices=["A", "B", "H", "C", "D", "E"],
                     help="workload (H is the headline; the others are reported for comparison)")
+    ap.add_argument("--samples", type=int, default=None,
+                    help="override the workload's sample count (same generator and seed)")
     ap.add_argument("--clusmethod", choices=["none", "gmm", "kmeans"], default=None,
                     help="override the workload's clustering method (config C runs both)")
     ap.add_argument("--pearson-fp64", action="store_true",
@@ -426,7 +428,7 @@ def main():
     if args.clusmethod:
         flags["clusmethod"] = args.clusmethod
     t = time.time()
-    x = gemgen.make(args.config)
+    x = gemgen.make(args.config, S=args.samples)
     G, S = x.shape
     log("generated %dx%d GEM in %.1fs" % (G, S, time.time() - t))
     total_pairs = native.pair_count(G)
@@ -481,6 +483,8 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS,
                 "bytes_formula": "8*S + 1 + S*[K>1] + 4*maxclus per pair"}
     whole_tflops = flops_total / secs / 1e12
+    # the counter profile covers the headline workload as generated
+    profiled = args.config == "H" and not args.samples and not args.clusmethod and not args.pearson_fp64
 
     cpu = None
     fp64_err = None
@@ -536,7 +540,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": WORKLOADS[args.config] + (
-                "; clustering overridden: %s" % args.clusmethod if args.clusmethod else ""),
+                "; clustering overridden: %s" % args.clusmethod if args.clusmethod else "") + (
+                "; samples overridden: %d" % args.samples if args.samples else ""),
             "pairs_per_step": BLOCK,
             "step": "one 32768-pair work block",
             "form": "persistent",
@@ -548,11 +553,11 @@ def main():
         "roofline": dict(roof, **{
             "kernel": ("k_pearson_lanes (one pair per lane, persistent form)" if dom == -1
                        else "k_pairs<P=%d,KMAX=5> (persistent form)" % dom),
-            "traffic": TRAFFIC_BYTES_PER_LAUNCH if args.config == "H" else None,
-            "traffic_source": TRAFFIC_SOURCE if args.config == "H" else None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH if profiled else None,
+            "traffic_source": TRAFFIC_SOURCE if profiled else None,
             "valu_issue": ({"insts_per_s": VALU_ISSUE_RATE, "ceiling": VALU_ISSUE_CEILING,
                             "frac": VALU_ISSUE_RATE / VALU_ISSUE_CEILING,
-                            "source": TRAFFIC_SOURCE} if args.config == "H" else None),
+                            "source": TRAFFIC_SOURCE} if profiled else None),
             "whole_job_tflops": whole_tflops,
         }),
         "cpu_baseline": cpu,

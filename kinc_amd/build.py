@@ -26,10 +26,17 @@ ARCH = os.environ.get("KINC_OFFLOAD_ARCH", "gfx950")
 # spills only in the cold paths (the K-means and memory-store branches) and
 # runs the GMM/Spearman workload 1.45x faster than the spill-free 2
 PAIR_WAVES = int(os.environ.get("KINC_PAIR_WAVES", "4"))
+# instances of at most 4 register slots (N <= 256) need fewer registers and,
+# with the LDS of up to about 400 samples, fit 5 waves per SIMD: measured
+# 1.11 M against 1.03 M pairs/s at 256 samples
+PAIR_WAVES_SMALL_P = int(os.environ.get("KINC_PAIR_WAVES_SMALL_P", "5"))
+
+
+def pair_waves(p):
+    return PAIR_WAVES_SMALL_P if 1 <= p <= 4 else PAIR_WAVES
 
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
-    "-DKINC_PAIR_WAVES=%d" % PAIR_WAVES,
     "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -64,9 +71,10 @@ def _compile(src, defines=(), tag=None):
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     inst = sorted(INSTANCES, key=lambda kp: -(kp[0] * (kp[1] or 4)))  # longest first
-    jobs = [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p),
+    jobs = [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p,
+                                    "KINC_PAIR_WAVES=%d" % pair_waves(p)),
              "pairs_k%d_p%d" % (k, p)) for k, p in inst]
-    jobs += [(src, (), None) for src in SOURCES]
+    jobs += [(src, ("KINC_PAIR_WAVES=%d" % PAIR_WAVES,), None) for src in SOURCES]
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))
