@@ -25,15 +25,7 @@ ARCH = os.environ.get("KINC_OFFLOAD_ARCH", "gfx950")
 # waves per SIMD the pair kernels are register-budgeted for: 4 (128 VGPRs)
 # spills only in the cold paths (the K-means and memory-store branches) and
 # runs the GMM/Spearman workload 1.45x faster than the spill-free 2
-PAIR_WAVES = int(os.environ.get("KINC_PAIR_WAVES", "4"))
-# instances of at most 4 register slots (N <= 256) need fewer registers and,
-# with the LDS of up to about 400 samples, fit 5 waves per SIMD: measured
-# 1.11 M against 1.03 M pairs/s at 256 samples
-PAIR_WAVES_SMALL_P = int(os.environ.get("KINC_PAIR_WAVES_SMALL_P", "5"))
-
-
-def pair_waves(p):
-    return PAIR_WAVES_SMALL_P if 1 <= p <= 4 else PAIR_WAVES
+PAIR_WAVES = 4
 
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
@@ -55,6 +47,13 @@ SOURCES = [
 # 5 clusters, 512 for up to 8)
 INSTANCES = ([(5, p) for p in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16)]
              + [(8, p) for p in (0, 2, 4, 8)])
+# the instances of at most 6 slots (N <= 384) are built a second time for 5
+# waves per SIMD (96 VGPRs); the library uses those when the per-wave LDS
+# leaves room for 20 waves per CU (about 400 samples or fewer): 1.11 M
+# against 1.03 M pairs/s at 256 samples, 726k against 696k at 384, while at
+# 500 samples, where LDS allows 16, mixing them into the concurrent bins
+# cost time
+INSTANCES_W5 = [(5, p) for p in (1, 2, 3, 4, 5, 6)] + [(8, p) for p in (2, 4)]
 CLI_SOURCE = "kinc_cli.cpp"
 CLI = os.path.join(HERE, "kinc")
 
@@ -72,8 +71,11 @@ def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     inst = sorted(INSTANCES, key=lambda kp: -(kp[0] * (kp[1] or 4)))  # longest first
     jobs = [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p,
-                                    "KINC_PAIR_WAVES=%d" % pair_waves(p)),
+                                    "KINC_PAIR_WAVES=%d" % PAIR_WAVES),
              "pairs_k%d_p%d" % (k, p)) for k, p in inst]
+    jobs += [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p,
+                                     "KINC_PAIR_WAVES=5"),
+              "pairs_k%d_p%d_w5" % (k, p)) for k, p in INSTANCES_W5]
     jobs += [(src, ("KINC_PAIR_WAVES=%d" % PAIR_WAVES,), None) for src in SOURCES]
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with ThreadPoolExecutor(max_workers=workers) as ex:

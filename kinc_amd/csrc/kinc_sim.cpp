@@ -27,10 +27,15 @@
 #define KINC_FOR_EACH_INSTANCE(X)                                                   \
    X(5, 0) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(5, 10)   \
    X(5, 12) X(5, 14) X(5, 16) X(8, 0) X(8, 2) X(8, 4) X(8, 8)
+/* small instances also built for 5 waves per SIMD */
+#define KINC_FOR_EACH_INSTANCE_W5(X) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(8, 2) X(8, 4)
 
 extern "C" {
 #define KINC_DECL(K, P) const void* kinc_pair_kernel_k##K##_p##P();
 KINC_FOR_EACH_INSTANCE(KINC_DECL)
+#undef KINC_DECL
+#define KINC_DECL(K, P) const void* kinc_pair_kernel_k##K##_p##P##_w5();
+KINC_FOR_EACH_INSTANCE_W5(KINC_DECL)
 #undef KINC_DECL
 const void* kinc_kernel_count_valid();
 const void* kinc_kernel_chunk_max();
@@ -71,9 +76,17 @@ std::vector<int> slot_classes(int S, int kmax)
    return out;
 }
 
-/* The pair kernel for (P, KMAX); one instance serves both forms. */
-const void* kinc_select_pair_kernel(int P, int kmax)
+/* The pair kernel for (P, KMAX); one instance serves both forms.  With w5,
+ * the 5-waves-per-SIMD build where one exists. */
+const void* kinc_select_pair_kernel(int P, int kmax, bool w5 = false)
 {
+#define KINC_PICK(K, PP)                                                      \
+   if ( kmax == K && P == PP ) return kinc_pair_kernel_k##K##_p##PP##_w5();
+   if ( w5 )
+   {
+      KINC_FOR_EACH_INSTANCE_W5(KINC_PICK)
+   }
+#undef KINC_PICK
 #define KINC_PICK(K, PP)                                                      \
    if ( kmax == K && P == PP ) return kinc_pair_kernel_k##K##_p##PP();
    KINC_FOR_EACH_INSTANCE(KINC_PICK)
@@ -135,6 +148,9 @@ struct kinc_sim
    std::string err;
    /* kernel timing */
    bool profiling = false;
+   /* the per-wave LDS leaves room for 20 waves per CU: small instances run
+    * their 5-waves-per-SIMD builds */
+   bool w5 = false;
    struct Timed { int P; hipEvent_t a, b; };
    std::vector<Timed> timed;
    std::vector<hipEvent_t> event_pool;
@@ -357,6 +373,7 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
       a.lds_tile = a.lds_u;
       off += std::max(tile_bytes, sort_bytes);
       h->lds_bytes = align16(off);
+      h->w5 = h->lds_bytes * 20 <= (size_t)160 * 1024;
    }
    else
    {
@@ -535,7 +552,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
    {
       /* the slot count S needs (or the memory store) for every pair */
       const int P = h->classes.back();
-      const void* fn = kinc_select_pair_kernel(P, h->kmax);
+      const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
       if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       a.list = nullptr;
       for ( int first = 0; first < cnt; first += h->p.ksize )
@@ -565,7 +582,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       /* no fits: the slot count does not matter and every pair costs about
        * the same, so resident waves stride over the whole block in one launch */
       const int P = h->classes[0];
-      const void* fn = kinc_select_pair_kernel(P, h->kmax);
+      const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
       if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
       HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
@@ -605,7 +622,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
    {
       const int P = h->classes[b];
       hipStream_t sb = h->aux[(h->nbins - 1 - b) % kinc_sim::kAux];
-      const void* fn = kinc_select_pair_kernel(P, h->kmax);
+      const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
       if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
       HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));

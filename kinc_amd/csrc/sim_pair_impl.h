@@ -39,12 +39,13 @@
 #define WAVE 64
 #define TILE 64
 
-/* One wave per workgroup; the second bound asks for at least KINC_PAIR_WAVES
- * waves per SIMD, which caps the VGPR budget of the pair kernels (512 / W). */
+/* One wave per workgroup; the second bound of a pair kernel asks for at
+ * least W waves per SIMD, which caps its VGPR budget (512 / W).  W is a
+ * template parameter of the kernel, so instances built for different W are
+ * different kernels. */
 #ifndef KINC_PAIR_WAVES
 #define KINC_PAIR_WAVES 4
 #endif
-#define KINC_PAIR_BOUNDS __launch_bounds__(64, KINC_PAIR_WAVES)
 
 namespace {
 
@@ -1650,8 +1651,8 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
  * pair ids from a device queue until it is empty, so every wave reaches the
  * exit test.
  */
-template <int P, int KMAX>
-__global__ KINC_PAIR_BOUNDS void k_pairs(SimArgs a)
+template <int P, int KMAX, int W>
+__global__ __launch_bounds__(64, W) void k_pairs(SimArgs a)
 {
    if ( a.list == nullptr )
    {

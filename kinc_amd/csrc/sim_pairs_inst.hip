@@ -1,9 +1,10 @@
 /*
  * sim_pairs_inst.hip — one pair-kernel instance per translation unit.
  *
- * Compiled once per (KINC_INST_P, KINC_INST_KMAX) by kinc_amd/build.py so
- * that the instances build in parallel; each exports a getter named
- * kinc_pair_kernel_k<KMAX>_p<P> returning the kernel's host handle.
+ * Compiled once per (KINC_INST_P, KINC_INST_KMAX, KINC_PAIR_WAVES) by
+ * kinc_amd/build.py so that the instances build in parallel; each exports a
+ * getter named kinc_pair_kernel_k<KMAX>_p<P> (4 waves per SIMD) or
+ * kinc_pair_kernel_k<KMAX>_p<P>_w5 (5) returning the kernel's host handle.
  */
 #include "sim_pair_impl.h"
 
@@ -17,7 +18,14 @@
 #define KINC_CAT2(a, b, c, d) a##b##c##d
 #define KINC_CAT(a, b, c, d) KINC_CAT2(a, b, c, d)
 
+#if KINC_PAIR_WAVES == 4
 extern "C" const void* KINC_CAT(kinc_pair_kernel_k, KINC_INST_KMAX, _p, KINC_INST_P)()
+#else
+/* the 5-waves-per-SIMD build of a small instance */
+#define KINC_CAT3(a, b, c, d, e) a##b##c##d##e
+#define KINC_CATW(a, b, c, d, e) KINC_CAT3(a, b, c, d, e)
+extern "C" const void* KINC_CATW(kinc_pair_kernel_k, KINC_INST_KMAX, _p, KINC_INST_P, _w5)()
+#endif
 {
-   return (const void*)&k_pairs<KINC_INST_P, KINC_INST_KMAX>;
+   return (const void*)&k_pairs<KINC_INST_P, KINC_INST_KMAX, KINC_PAIR_WAVES>;
 }
