@@ -75,6 +75,27 @@ def test_create_rejects_bad_params_before_device(native, bad, code):
     assert e.value.code == code
 
 
+@pytest.mark.parametrize("field,value", [("pearson_fp64", 2), ("pearson_fp64", -1),
+                                         ("batch_capacity", -1), ("batch_capacity", (1 << 24) + 1)])
+def test_create_rejects_bad_extension_fields(native, field, value):
+    # the fields the reference does not have are range-checked too
+    import numpy as np
+    p = native.make_params()
+    setattr(p, field, value)
+    with pytest.raises(native.KincError) as e:
+        native.Engine(np.zeros((4, 8), np.float32), p)
+    assert e.value.code == _native.KINC_E_INVALID
+
+
+def test_create_rejects_nonzero_reserved(native):
+    import numpy as np
+    p = native.make_params()
+    p.reserved[1] = 7
+    with pytest.raises(native.KincError) as e:
+        native.Engine(np.zeros((4, 8), np.float32), p)
+    assert e.value.code == _native.KINC_E_INVALID
+
+
 def test_rmt_default_params_match_reference(native):
     # src/core/rmt.h:31-40
     from kinc_amd import rmt
