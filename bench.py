@@ -103,6 +103,36 @@ def block_flops(K, labels, iters, corr_method, clus="gmm"):
     return f, N
 
 
+def block_other_work(K, labels, iters, corr_method, minsamp, clus="gmm"):
+    """Work the flop count leaves out: GMM exponentials N*K*(T + 2*T_M) and
+    logarithms N*(T + 1) per fit, and the Spearman sorts' compare-exchanges,
+    2 * (n2/2) * L(L+1)/2 per cluster of n >= minsamp points with
+    n2 = next power of two of n (at least 2) and L = log2(n2)."""
+    N = labels.shape[1] - np.count_nonzero((labels == -9) | (labels == -6), axis=1)
+    exps = logs = cx = 0
+    for q in range(K.shape[0]):
+        if clus == "gmm":
+            for k in range(iters.shape[1]):
+                it = int(iters[q, k])
+                if it == 0:
+                    continue
+                T = it & 0xFFF
+                TM = T - 1 if (it >> 12) & 1 else T
+                exps += int(N[q]) * (k + 1) * (T + 2 * TM)
+                logs += int(N[q]) * (T + 1)
+        if corr_method == "spearman":
+            for k in range(int(K[q])):
+                n = int(np.count_nonzero(labels[q] == k))
+                if n < minsamp:
+                    continue
+                n2 = 2
+                while n2 < n:
+                    n2 *= 2
+                L = n2.bit_length() - 1
+                cx += 2 * (n2 // 2) * L * (L + 1) // 2
+    return exps, logs, cx
+
+
 def slots_for(n, S=500):
     """Register-slot class of a pair with n valid samples (the persistent
     form's bin; classes above the one S needs are not built)."""
@@ -379,6 +409,8 @@ def main():
                     help="time the CPU oracle on one thread only")
     ap.add_argument("--config", default="H", choices=["A", "B", "H", "C", "D", "E"],
                     help="workload (H is the headline; the others are reported for comparison)")
+    ap.add_argument("--set", action="append", default=[], metavar="FLAG=VALUE",
+                    help="override one analytic flag of the workload (e.g. preout=false, minsamp=20)")
     ap.add_argument("--samples", type=int, default=None,
                     help="override the workload's sample count (same generator and seed)")
     ap.add_argument("--clusmethod", choices=["none", "gmm", "kmeans"], default=None,
@@ -427,6 +459,18 @@ def main():
         flags["pearson_fp64"] = True
     if args.clusmethod:
         flags["clusmethod"] = args.clusmethod
+    for kv in args.set:
+        key, _, val = kv.partition("=")
+        if val.lower() in ("true", "false"):
+            flags[key] = val.lower() == "true"
+        else:
+            try:
+                flags[key] = int(val)
+            except ValueError:
+                try:
+                    flags[key] = float(val)
+                except ValueError:
+                    flags[key] = val
     t = time.time()
     x = gemgen.make(args.config, S=args.samples)
     G, S = x.shape
@@ -483,8 +527,24 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS,
                 "bytes_formula": "8*S + 1 + S*[K>1] + 4*maxclus per pair"}
     whole_tflops = flops_total / secs / 1e12
+    other = [0, 0, 0]
+    for i in range(len(blocks)):
+        e, l_, c = block_other_work(out["K"][i], out["labels"][i], out["iters"][i],
+                                    flags["corrmethod"], flags.get("minsamp", 30),
+                                    flags.get("clusmethod", "none"))
+        other[0] += e
+        other[1] += l_
+        other[2] += c
+    other_work = {
+        "gmm_exp_per_s": other[0] / secs, "gmm_log_per_s": other[1] / secs,
+        "spearman_compare_exchanges_per_s": other[2] / secs,
+        "per_pair": {"gmm_exp": other[0] / pairs, "gmm_log": other[1] / pairs,
+                     "spearman_compare_exchanges": other[2] / pairs},
+        "note": "software exp/log (15 and about 30 VALU instructions), counted apart from the flops",
+    }
     # the counter profile covers the headline workload as generated
-    profiled = args.config == "H" and not args.samples and not args.clusmethod and not args.pearson_fp64
+    profiled = (args.config == "H" and not args.samples and not args.clusmethod
+                and not args.pearson_fp64 and not args.set)
 
     cpu = None
     fp64_err = None
@@ -541,7 +601,8 @@ def main():
         "config": {
             "workload": WORKLOADS[args.config] + (
                 "; clustering overridden: %s" % args.clusmethod if args.clusmethod else "") + (
-                "; samples overridden: %d" % args.samples if args.samples else ""),
+                "; samples overridden: %d" % args.samples if args.samples else "") + (
+                "; flags overridden: %s" % ", ".join(args.set) if args.set else ""),
             "pairs_per_step": BLOCK,
             "step": "one 32768-pair work block",
             "form": "persistent",
@@ -560,6 +621,7 @@ def main():
                             "source": TRAFFIC_SOURCE} if profiled else None),
             "whole_job_tflops": whole_tflops,
         }),
+        "other_work": other_work,
         "cpu_baseline": cpu,
         "corr_error_vs_fp64": fp64_err,
         "serving": serving_obj,

@@ -270,16 +270,17 @@ __device__ __forceinline__ void bitonic_sort(float* key, float* pay, int size, b
 {
    const int lane = lane_id();
    const int half = size >> 1;
-   for ( int ob = 2; ob <= size; ob <<= 1 )
+   /* size, ob, ib and t are powers of two: the divisions are shifts */
+   for ( int lob = 1; (1 << lob) <= size; ++lob )
    {
-      const int obh = ob >> 1;
-      for ( int ib = ob; ib >= 2; ib >>= 1 )
+      const int lobh = lob - 1;                /* log2(ob / 2) */
+      for ( int lib = lob; lib >= 1; --lib )
       {
-         const int t = ib >> 1;
+         const int t = 1 << (lib - 1);
          for ( int i = lane; i < half; i += WAVE )
          {
-            int dir = (i / obh) & 1;
-            int aa = (i / t) * ib + (i % t);
+            int dir = (i >> lobh) & 1;
+            int aa = ((i >> (lib - 1)) << lib) + (i & (t - 1));
             int bb = aa + t;
             float ka = key[aa];
             float kb = key[bb];
