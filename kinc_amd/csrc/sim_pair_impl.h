@@ -823,8 +823,10 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          mx = (vk > mx) ? vk : mx;
       }
       float sum = 0.0f;
+      /* exp(v_k - mx) on lane k, all at once; summed in k order */
+      const float ek = kinc_expf(v - mx);
 #pragma unroll
-      for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf(bcast(v, k) - mx);
+      for ( int k = 0; k < K; ++k ) sum = sum + bcast(ek, k);
       const float logGammaSum = mx + kinc_logf(sum);
 
       const float Gam = kinc_expf(logGamma);
