@@ -124,3 +124,40 @@ def test_block_parity(case, form):
         eng.close()
     print("case", name, form, "pairs", count, "K hist", np.bincount(want[0].astype(np.int64) + 0))
     assert_block_equal(got, want, S, where="%s/%s" % (name, form))
+
+
+def _with_infinities(x, seed):
+    """Expression data as a log transform of zero counts leaves it: a fifth
+    of the genes hold a few -inf entries, one gene a +inf, and every seventh
+    gene is constant."""
+    rng = np.random.default_rng(seed)
+    x = x.copy()
+    G, S = x.shape
+    for g in range(1, G, 5):
+        x[g, rng.choice(S, 3, replace=False)] = -np.inf
+    x[2, rng.integers(0, S)] = np.inf
+    x[::7] = 4.0
+    return x
+
+
+@pytest.mark.parametrize("form", ["plain", "persistent"])
+@pytest.mark.parametrize("flags", [
+    dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True),
+    dict(clusmethod="gmm", corrmethod="pearson", crit="ICL"),
+    dict(clusmethod="kmeans", corrmethod="pearson", crit="BIC", postout=True),
+    dict(clusmethod="none", corrmethod="pearson"),
+    dict(clusmethod="none", corrmethod="spearman"),
+], ids=["gmm_spearman", "gmm_pearson_icl", "kmeans_pearson", "none_pearson", "none_spearman"])
+def test_parity_with_infinities_and_constant_genes(flags, form):
+    x = _with_infinities(gemgen.modules(28, 180, 48, nan_rate=0.05), 49)
+    G, S = x.shape
+    count = _native.pair_count(G)
+    p = _native.make_params(form=form, ksize=1000, **flags)
+    want = oracle.execute_block(p, x, 0, count)
+    eng = _native.Engine(x, p)
+    try:
+        got = eng.execute(0, count)
+    finally:
+        eng.close()
+    print("infinities", flags, form, "pairs", count, flush=True)
+    assert_block_equal(got, want, S, where="infinities %s/%s" % (flags, form))
