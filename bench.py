@@ -8,8 +8,10 @@ one KINC work block of 32,768 consecutive gene pairs; the steps sample the
 199,990,000-pair space at every 97th block.  The expression matrix is
 resident in HBM before timing starts and outputs stay on the device.
 
-The JSON line reports pairs/s of the batched persistent form (`value`), of
-the plain one-launch-per-ksize form (`plain_value`), the FP32 VALU roofline
+The JSON line reports pairs/s of the batched persistent form on `--workers`
+engine handles taking blocks in turn (`value`, default 2), of the plain
+one-launch-per-ksize form on one handle (`plain_value`) and on the same
+handles (`plain_value_same_workers`), the FP32 VALU roofline
 fraction of the dominant pair kernel, timed with HIP events on the stream it
 runs on, and the CPU oracle timed on a bounded sample of the same pairs on
 this host (which also re-checks bit-exact parity on that sample).
@@ -142,12 +144,15 @@ def slots_for(n, S=500):
     return 0
 
 
-def run_form(native, x, flags, form, blocks, warmup, torch):
+def run_form(native, x, flags, form, blocks, warmup, torch, workers=2):
+    """Time the blocks on `workers` engine handles taking blocks in turn (the
+    reference's several workers per device): each handle has its own stream,
+    so a block's bins start while the previous block's last pairs finish."""
     import ctypes  # noqa: F401
     G, S = x.shape
     total_pairs = native.pair_count(G)
     p = native.make_params(form=form, **flags)
-    eng = native.Engine(x, p)
+    engs = [native.Engine(x, p) for _ in range(workers)]
     MC = p.max_clusters
     nsteps = len(blocks)
     dev = torch.device("cuda:0")
@@ -159,30 +164,38 @@ def run_form(native, x, flags, form, blocks, warmup, torch):
     def run(i, b):
         start = b * BLOCK
         count = min(BLOCK, total_pairs - start)
-        eng.execute_device(start, count, dK[i].data_ptr(), dL[i].data_ptr(), dC[i].data_ptr(),
-                           dI[i].data_ptr())
+        engs[i % workers].execute_device(start, count, dK[i].data_ptr(), dL[i].data_ptr(),
+                                         dC[i].data_ptr(), dI[i].data_ptr())
         return count
 
-    for i in range(warmup):
+    for i in range(max(warmup, workers)):
         run(i % nsteps, blocks[i % nsteps])
-    eng.synchronize()
+    for e in engs:
+        e.synchronize()
     torch.cuda.synchronize()
-    eng.set_profiling(True)
-    eng.profile_read()
+    for e in engs:
+        e.set_profiling(True)
+        e.profile_read()
     t0 = time.perf_counter()
     pairs = 0
     for i, b in enumerate(blocks):
         pairs += run(i, b)
         log("  %s step %d/%d queued (block %d)" % (form, i + 1, nsteps, b))
-    eng.synchronize()
+    for e in engs:
+        e.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = eng.profile_read()
-    eng.set_profiling(False)
+    prof = {}
+    for e in engs:
+        for P, (n, ms) in e.profile_read().items():
+            n0, ms0 = prof.get(P, (0, 0.0))
+            prof[P] = (n0 + n, ms0 + ms)
+        e.set_profiling(False)
     out = dict(
         K=dK.cpu().numpy(), labels=dL.cpu().numpy(), corr=dC.cpu().numpy(),
         iters=dI.cpu().numpy().view(np.uint16))
-    eng.close()
+    for e in engs:
+        e.close()
     return pairs, t1 - t0, prof, out
 
 
@@ -409,6 +422,8 @@ def main():
                     help="time the CPU oracle on one thread only")
     ap.add_argument("--config", default="H", choices=["A", "B", "H", "C", "D", "E"],
                     help="workload (H is the headline; the others are reported for comparison)")
+    ap.add_argument("--workers", type=int, default=2,
+                    help="engine handles taking blocks in turn on one GPU (each its own stream)")
     ap.add_argument("--set", action="append", default=[], metavar="FLAG=VALUE",
                     help="override one analytic flag of the workload (e.g. preout=false, minsamp=20)")
     ap.add_argument("--samples", type=int, default=None,
@@ -481,15 +496,26 @@ def main():
     if len(blocks) < args.steps:
         blocks = list(range(min(args.steps, nblocks)))
 
-    pairs, secs, prof, out = run_form(native, x, flags, "persistent", blocks, args.warmup, torch)
+    pairs, secs, prof, out = run_form(native, x, flags, "persistent", blocks, args.warmup, torch,
+                                      args.workers)
     value = pairs / secs
     log("persistent: %d pairs in %.3fs -> %.0f pairs/s; kernels %s" % (pairs, secs, value, prof))
 
     plain_value = None
+    plain_value_workers = None
     if not args.no_plain:
-        pp, ps, pprof, pout = run_form(native, x, flags, "plain", blocks, args.warmup, torch)
+        # the plain form as the reference runs its blocks: one launch per
+        # ksize sub-batch, one block after another on one handle
+        pp, ps, pprof, pout = run_form(native, x, flags, "plain", blocks, args.warmup, torch, 1)
         plain_value = pp / ps
         log("plain: %d pairs in %.3fs -> %.0f pairs/s" % (pp, ps, plain_value))
+        if args.workers > 1:
+            # and with the same worker handles as the persistent form
+            pw, pws, _, _ = run_form(native, x, flags, "plain", blocks, args.warmup, torch,
+                                     args.workers)
+            plain_value_workers = pw / pws
+            log("plain, %d workers: %d pairs in %.3fs -> %.0f pairs/s"
+                % (args.workers, pw, pws, plain_value_workers))
         same = all(np.array_equal(out[k], pout[k]) for k in ("K", "labels", "iters")) and \
             np.array_equal(np.nan_to_num(out["corr"], nan=9.0).view(np.uint32),
                            np.nan_to_num(pout["corr"], nan=9.0).view(np.uint32))
@@ -607,10 +633,13 @@ def main():
             "step": "one 32768-pair work block",
             "form": "persistent",
             "parallelism": "1 GPU",
+            "workers": args.workers,
             "pearson_fp64": bool(args.pearson_fp64),
         },
         "plain_value": plain_value,
         "batched_vs_plain": (value / plain_value) if plain_value else None,
+        "plain_value_same_workers": plain_value_workers,
+        "batched_vs_plain_same_workers": (value / plain_value_workers) if plain_value_workers else None,
         "roofline": dict(roof, **{
             "kernel": ("k_pearson_lanes (one pair per lane, persistent form)" if dom == -1
                        else "k_pairs<P=%d,KMAX=5> (persistent form)" % dom),

@@ -9,9 +9,9 @@
  *        [--minexpr F] [--minsamp N] [--minclus N] [--maxclus N]
  *        [--crit BIC|ICL] [--preout true|false] [--postout true|false]
  *        [--mincorr F] [--maxcorr F] [--ksize N]
- * plus three flags the reference does not have: --device N, --form
- * plain|persistent and --pearson-fp64 BOOL (Pearson sums and quotient in
- * double); and the
+ * plus flags the reference does not have: --device N, --form
+ * plain|persistent, --pearson-fp64 BOOL (Pearson sums and quotient in
+ * double) and --workers 1|2 (engine handles taking blocks in turn); and the
  * expression-matrix converters around it (importexpressionmatrix_input.cpp,
  * exportexpressionmatrix_input.cpp):
  *   kinc run import-emx --input A.txt --output A.emx [--nan TOKEN]
@@ -59,7 +59,7 @@ int usage()
            "       [--corrmethod pearson|spearman] [--minexpr F] [--minsamp N] [--minclus N]\n"
            "       [--maxclus N] [--crit BIC|ICL] [--preout BOOL] [--postout BOOL]\n"
            "       [--mincorr F] [--maxcorr F] [--ksize N] [--device N] [--form plain|persistent]\n"
-           "       [--pearson-fp64 BOOL]\n"
+           "       [--pearson-fp64 BOOL] [--workers 1|2]\n"
            "       kinc run import-emx --input TXT --output EMX [--nan TOKEN] [--samples N]\n"
            "       [--transform none|'natural logarithm'|'logarithm base 2'|'logarithm base 10']\n"
            "       [--threads N]\n"
@@ -228,6 +228,7 @@ int main(int argc, char** argv)
    kinc_sim_default_params(&p);
    std::string in, ccm, cmx;
    int device = 0;
+   int workers = 2;
    for ( int i = 3; i < argc; i += 2 )
    {
       if ( i + 1 >= argc ) return usage();
@@ -248,6 +249,7 @@ int main(int argc, char** argv)
       else if ( k == "--maxcorr" ) p.max_corr = strtof(v.c_str(), nullptr);
       else if ( k == "--ksize" ) p.ksize = atoi(v.c_str());
       else if ( k == "--device" ) device = atoi(v.c_str());
+      else if ( k == "--workers" ) workers = atoi(v.c_str());
       else if ( k == "--form" ) p.form = v == "plain" ? KINC_FORM_PLAIN : KINC_FORM_PERSISTENT;
       else if ( k == "--pearson-fp64" ) p.pearson_fp64 = parse_bool(v);
       else return usage();
@@ -255,6 +257,11 @@ int main(int argc, char** argv)
    if ( in.empty() || ccm.empty() || cmx.empty() )
    {
       fprintf(stderr, "Did not get valid input and/or output arguments.\n");
+      return 2;
+   }
+   if ( workers < 1 || workers > 2 )
+   {
+      fprintf(stderr, "--workers must be 1 or 2.\n");
       return 2;
    }
 
@@ -274,18 +281,25 @@ int main(int argc, char** argv)
    std::string meta(msize > 0 ? (size_t)msize : 0, '\0');
    if ( msize > 0 ) kinc_file_metadata(in.c_str(), &meta[0], msize);
 
-   kinc_sim* h = nullptr;
-   if ( kinc_sim_create(&p, emx.data(), G, S, device, &h) != KINC_OK )
+   /* blocks go in turn to `workers` engine handles, each with its own
+    * stream, so a block's bins start while the previous block's last pairs
+    * finish (as the reference runs several workers per device) */
+   kinc_sim* hs[2] = {nullptr, nullptr};
+   for ( int k = 0; k < workers; ++k )
    {
-      fprintf(stderr, "%s\n", kinc_sim_last_error(nullptr));
-      return 1;
+      if ( kinc_sim_create(&p, emx.data(), G, S, device, &hs[k]) != KINC_OK )
+      {
+         fprintf(stderr, "%s\n", kinc_sim_last_error(nullptr));
+         for ( int j = 0; j < k; ++j ) kinc_sim_destroy(hs[j]);
+         return 1;
+      }
    }
    kinc_similarity_writer* w = nullptr;
    if ( kinc_similarity_writer_open(ccm.c_str(), cmx.c_str(), G, S, p.max_clusters, p.corr_method,
                                     p.min_corr, p.max_corr, meta.empty() ? nullptr : meta.c_str(), &w) != KINC_OK )
    {
       fprintf(stderr, "%s\n", kinc_formats_last_error());
-      kinc_sim_destroy(h);
+      for ( int k = 0; k < workers; ++k ) kinc_sim_destroy(hs[k]);
       return 1;
    }
 
@@ -293,11 +307,15 @@ int main(int argc, char** argv)
    const int64_t B = KINC_WORK_BLOCK_SIZE;
    const int64_t nblocks = (total + B - 1) / B;
    const int MC = p.max_clusters;
-   hipStream_t stream;
+   hipStream_t streams[2] = {nullptr, nullptr};
    DevSet dev[2];
    HostSet host[2];
    hipEvent_t done[2];
-   bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess;
+   bool ok = hipSetDevice(device) == hipSuccess;
+   for ( int k = 0; ok && k < workers; ++k )
+   {
+      ok = hipStreamCreateWithFlags(&streams[k], hipStreamNonBlocking) == hipSuccess;
+   }
    for ( int s = 0; ok && s < 2; ++s )
    {
       ok = hipMalloc(&dev[s].K, B) == hipSuccess && hipMalloc(&dev[s].labels, B * S) == hipSuccess
@@ -317,6 +335,8 @@ int main(int argc, char** argv)
    auto t0 = std::chrono::steady_clock::now();
    auto submit = [&](int64_t b) -> bool {
       const int s = (int)(b & 1);
+      kinc_sim* h = hs[s % workers];
+      hipStream_t stream = streams[s % workers];
       const int64_t start = b * B;
       const int64_t count = std::min(B, total - start);
       if ( kinc_sim_execute_device(h, start, count, dev[s].K, want_labels ? dev[s].labels : nullptr,
@@ -376,7 +396,10 @@ int main(int argc, char** argv)
       (void)hipHostFree(host[s].corr);
       (void)hipEventDestroy(done[s]);
    }
-   (void)hipStreamDestroy(stream);
-   kinc_sim_destroy(h);
+   for ( int k = 0; k < workers; ++k )
+   {
+      (void)hipStreamDestroy(streams[k]);
+      kinc_sim_destroy(hs[k]);
+   }
    return rc;
 }

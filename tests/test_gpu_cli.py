@@ -90,3 +90,18 @@ def test_cli_pearson_fp64_option(tmp_path):
     d32.mkdir()
     c32, x32, _ = run_cli(d32, x, flags, "persistent")
     assert formats.read_file(x32)[1] != formats.read_file(cmx)[1]
+
+
+def test_cli_one_or_two_workers_same_files(tmp_path):
+    # blocks taken in turn by two engine handles give the same files as one
+    x = gemgen.modules(300, 90, seed=9, nan_rate=0.1)
+    flags = dict(clusmethod="gmm", corrmethod="pearson", crit="BIC", mincorr=0.3)
+    one = tmp_path / "one"
+    one.mkdir()
+    c1, x1, _ = run_cli(one, x, dict(flags, workers=1), "persistent")
+    two = tmp_path / "two"
+    two.mkdir()
+    c2, x2, _ = run_cli(two, x, dict(flags, workers=2), "persistent")
+    assert formats.read_file(c1)[1] == formats.read_file(c2)[1]
+    assert formats.read_file(x1)[1] == formats.read_file(x2)[1]
+    assert formats.read_pairwise(x1)[0]["pairs"] > 100
