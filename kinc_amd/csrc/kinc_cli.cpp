@@ -310,7 +310,7 @@ int main(int argc, char** argv)
    hipStream_t streams[2] = {nullptr, nullptr};
    DevSet dev[2];
    HostSet host[2];
-   hipEvent_t done[2];
+   hipEvent_t done[2] = {nullptr, nullptr};
    bool ok = hipSetDevice(device) == hipSuccess;
    for ( int k = 0; ok && k < workers; ++k )
    {
@@ -324,10 +324,11 @@ int main(int argc, char** argv)
         && hipHostMalloc(&host[s].corr, sizeof(float) * B * MC, 0) == hipSuccess
         && hipEventCreate(&done[s]) == hipSuccess;
    }
+   int rc = 0;
    if ( !ok )
    {
       fprintf(stderr, "device allocation failed\n");
-      return 1;
+      rc = 1;
    }
 
    /* without clustering every pair has K = 1 and the writer needs no labels */
@@ -351,8 +352,7 @@ int main(int argc, char** argv)
           && hipMemcpyAsync(host[s].corr, dev[s].corr, sizeof(float) * count * MC, hipMemcpyDeviceToHost, stream) == hipSuccess
           && hipEventRecord(done[s], stream) == hipSuccess;
    };
-   int rc = 0;
-   if ( nblocks > 0 && !submit(0) ) rc = 1;
+   if ( rc == 0 && nblocks > 0 && !submit(0) ) rc = 1;
    for ( int64_t b = 0; rc == 0 && b < nblocks; ++b )
    {
       const int s = (int)(b & 1);
@@ -386,6 +386,9 @@ int main(int argc, char** argv)
       fprintf(stderr, "similarity: %lld pairs in %lld blocks, %.3f s (%.0f pairs/s)\n",
               (long long)total, (long long)nblocks, secs, total / secs);
    }
+   /* after an error blocks may still be in flight on the streams */
+   for ( int k = 0; k < workers; ++k )
+      if ( streams[k] ) (void)hipStreamSynchronize(streams[k]);
    for ( int s = 0; s < 2; ++s )
    {
       (void)hipFree(dev[s].K);
@@ -394,11 +397,11 @@ int main(int argc, char** argv)
       (void)hipHostFree(host[s].K);
       (void)hipHostFree(host[s].labels);
       (void)hipHostFree(host[s].corr);
-      (void)hipEventDestroy(done[s]);
+      if ( done[s] ) (void)hipEventDestroy(done[s]);
    }
    for ( int k = 0; k < workers; ++k )
    {
-      (void)hipStreamDestroy(streams[k]);
+      if ( streams[k] ) (void)hipStreamDestroy(streams[k]);
       kinc_sim_destroy(hs[k]);
    }
    return rc;
