@@ -221,9 +221,10 @@ def test_gmm_singular_covariance_fails():
     assert not ok and st[3] == 1
 
 
-def test_gmm_logl_nondecreasing_fp64_check():
-    """EM never decreases the likelihood in exact arithmetic; in FP32 the
-    trajectory may wobble by rounding only."""
+def test_gmm_two_overlapping_clusters():
+    """Two overlapping clusters: the fit succeeds after at least 3 E steps
+    with labels 0 and 1 only (the EM trajectory itself is checked against a
+    double-precision EM in test_oracle_em_fp64.py)."""
     rng = np.random.default_rng(5)
     X = np.concatenate([rng.normal([0, 0], 1, (100, 2)), rng.normal([3, 3], 1, (100, 2))]).astype(np.float32)
     ok, labels, logL, _, st = oracle.gmm_fit(X, 2)
