@@ -406,7 +406,19 @@ __device__ __forceinline__ float chain_add(const float* tile, int stride, int nc
       }
       else
       {
-         for ( int j = 0; j < len; ++j ) acc = acc + row[j];
+         /* the partial last tile: whole groups of four, then the rest, in
+          * the same index order */
+         const float4* r4 = (const float4*)row;
+         const int n4 = len >> 2;
+         for ( int j = 0; j < n4; ++j )
+         {
+            float4 v = r4[j];
+            acc = acc + v.x;
+            acc = acc + v.y;
+            acc = acc + v.z;
+            acc = acc + v.w;
+         }
+         for ( int j = n4 << 2; j < len; ++j ) acc = acc + row[j];
       }
    }
    return acc;
