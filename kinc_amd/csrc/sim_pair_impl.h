@@ -1205,7 +1205,18 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
       float Ssum = 0.0f;
       if ( lane < KINC_MIN_ACTIVE_LANES )   /* identical sums; lane 0's is used */
       {
-         for ( int j = 0; j < kbase; ++j ) Ssum = Ssum + seq[j];
+         /* four values per LDS read, added in index order */
+         const float4* s4 = (const float4*)seq;
+         const int n4 = kbase >> 2;
+         for ( int j = 0; j < n4; ++j )
+         {
+            const float4 v = s4[j];
+            Ssum = Ssum + v.x;
+            Ssum = Ssum + v.y;
+            Ssum = Ssum + v.z;
+            Ssum = Ssum + v.w;
+         }
+         for ( int j = n4 << 2; j < kbase; ++j ) Ssum = Ssum + seq[j];
       }
       const float logL = -bcast(Ssum, 0);
       wave_sync(L.lgbuf != nullptr);
