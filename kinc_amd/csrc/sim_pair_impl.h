@@ -1393,32 +1393,52 @@ __device__ __forceinline__ void pearson_all(const SimArgs& a, const Lds& L, int 
    {
       const int k = lane / 5;
       const int t = lane - 5 * k;
+      /* four samples per pass: one 4-byte read of their labels and two
+       * 16-byte reads of their points (both arrays start 16-byte aligned),
+       * added in index order */
+      const int S4 = S & ~3;
       if ( !a.pearson_fp64 )
       {
-         for ( int i = 0; i < S; ++i )
-         {
-            int8_t l = L.lab[i];
-            float2 v = L.X[i];
-            float term = (t == 0) ? v.x
-                       : (t == 1) ? v.y
-                       : (t == 2) ? v.x * v.x
-                       : (t == 3) ? v.y * v.y
-                       : v.x * v.y;
+         auto add = [&](int8_t l, float x, float y) {
+            float term = (t == 0) ? x
+                       : (t == 1) ? y
+                       : (t == 2) ? x * x
+                       : (t == 3) ? y * y
+                       : x * y;
             acc = (l == k) ? acc + term : acc;
+         };
+         for ( int i = 0; i < S4; i += 4 )
+         {
+            const uint32_t l4 = *(const uint32_t*)(L.lab + i);
+            const float4 va = *(const float4*)(L.X + i);
+            const float4 vb = *(const float4*)(L.X + i + 2);
+            add((int8_t)l4, va.x, va.y);
+            add((int8_t)(l4 >> 8), va.z, va.w);
+            add((int8_t)(l4 >> 16), vb.x, vb.y);
+            add((int8_t)(l4 >> 24), vb.z, vb.w);
          }
+         for ( int i = S4; i < S; ++i ) add(L.lab[i], L.X[i].x, L.X[i].y);
       }
       else
       {
          /* term = f1 * f2, exact in double (two 24-bit significands) */
-         for ( int i = 0; i < S; ++i )
-         {
-            int8_t l = L.lab[i];
-            float2 v = L.X[i];
-            const float f1 = (t == 1 || t == 3) ? v.y : v.x;
-            const float f2 = (t < 2) ? 1.0f : (t == 2) ? v.x : v.y;
+         auto add = [&](int8_t l, float x, float y) {
+            const float f1 = (t == 1 || t == 3) ? y : x;
+            const float f2 = (t < 2) ? 1.0f : (t == 2) ? x : y;
             const double term = (double)f1 * (double)f2;
             dacc = (l == k) ? dacc + term : dacc;
+         };
+         for ( int i = 0; i < S4; i += 4 )
+         {
+            const uint32_t l4 = *(const uint32_t*)(L.lab + i);
+            const float4 va = *(const float4*)(L.X + i);
+            const float4 vb = *(const float4*)(L.X + i + 2);
+            add((int8_t)l4, va.x, va.y);
+            add((int8_t)(l4 >> 8), va.z, va.w);
+            add((int8_t)(l4 >> 16), vb.x, vb.y);
+            add((int8_t)(l4 >> 24), vb.z, vb.w);
          }
+         for ( int i = S4; i < S; ++i ) add(L.lab[i], L.X[i].x, L.X[i].y);
       }
    }
 #pragma unroll
