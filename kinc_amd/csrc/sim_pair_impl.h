@@ -383,6 +383,34 @@ __device__ __forceinline__ void copy_back(const Lds& L, int N)
  */
 #define KINC_MIN_ACTIVE_LANES 16
 
+/* Add the first len (<= TILE) values of one tile row to acc in index order:
+ * groups of four float4 reads are issued together (all inside the row's
+ * TILE values), and only the groups' values below len are added. */
+__device__ __forceinline__ float chain_row_var(const float* row, int len, float acc)
+{
+   const float4* r4 = (const float4*)row;
+   const int n4 = len >> 2;
+   for ( int g = 0; g < n4; g += 4 )
+   {
+      float4 v[4];
+#pragma unroll
+      for ( int u = 0; u < 4; ++u ) v[u] = r4[g + u];
+#pragma unroll
+      for ( int u = 0; u < 4; ++u )
+      {
+         if ( g + u < n4 )
+         {
+            acc = acc + v[u].x;
+            acc = acc + v[u].y;
+            acc = acc + v[u].z;
+            acc = acc + v[u].w;
+         }
+      }
+   }
+   for ( int j = n4 << 2; j < len; ++j ) acc = acc + row[j];
+   return acc;
+}
+
 /* Add row `lane` of the staged tile (len values) to acc in index order, for
  * lanes < nch; lanes nch..15 repeat the last row and their sums are unused. */
 __device__ __forceinline__ float chain_add(const float* tile, int stride, int nch, int len, float acc)
@@ -430,15 +458,7 @@ __device__ __forceinline__ float chain_add4(const float* tile, int stride, int n
    const int lane = lane_id();
    if ( lane < nch || lane < KINC_MIN_ACTIVE_LANES )
    {
-      const float4* r4 = (const float4*)(tile + (lane < nch ? lane : nch - 1) * stride);
-      for ( int j = 0; j < (len4 >> 2); ++j )
-      {
-         float4 v = r4[j];
-         acc = acc + v.x;
-         acc = acc + v.y;
-         acc = acc + v.z;
-         acc = acc + v.w;
-      }
+      acc = chain_row_var(tile + (lane < nch ? lane : nch - 1) * stride, len4, acc);
    }
    return acc;
 }
