@@ -23,8 +23,8 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("KINC_OFFLOAD_ARCH", "gfx950")
 
 # waves per SIMD the pair kernels are register-budgeted for: 4 (128 VGPRs)
-# spills only in the cold paths (the K-means and memory-store branches) and
-# runs the GMM/Spearman workload 1.45x faster than the spill-free 2
+# spills (a few reloads per EM iteration) and runs the GMM/Spearman workload
+# 1.45x faster than the spill-free 2
 PAIR_WAVES = 4
 
 HIP_FLAGS = [
