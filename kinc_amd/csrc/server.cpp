@@ -308,7 +308,11 @@ static bool launch_batch(kinc_server* s, Batch& b)
 static void retire_batch(kinc_server* s, Batch& b)
 {
    float ms = 0.0f;
-   if ( hipEventElapsedTime(&ms, b.start, b.done) == hipSuccess ) s->busy_s += ms * 1e-3;
+   if ( hipEventElapsedTime(&ms, b.start, b.done) == hipSuccess )
+   {
+      std::lock_guard<std::mutex> g(s->mu);   /* kinc_server_stats reads it */
+      s->busy_s += ms * 1e-3;
+   }
    s->pairs_done += b.count;
    for ( auto& sg : b.segs )
    {
