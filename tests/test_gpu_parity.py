@@ -41,6 +41,12 @@ CASES = [
     ("gmm_spearman_E_S5000", dict(kind="modules", G=10, S=5000, seed=6, nan_rate=0.10,
                                   degenerate=0.2),
      dict(clusmethod="gmm", corrmethod="spearman", crit="BIC"), 0, 45),
+    # the largest sample count the device takes (32768): memory-store fits
+    # of about 26,500 points, Spearman sums far past the int32 wrap
+    ("gmm_spearman_S32768_max", dict(kind="modules", G=3, S=32768, seed=31, nan_rate=0.10),
+     dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True), 0, 3),
+    ("pearson_none_S32768_max", dict(kind="latent", G=6, S=32768, seed=32, nan_rate=0.05),
+     dict(clusmethod="none", corrmethod="pearson"), 0, 15),
     # parameter corners and slot-count boundaries: a sample count at the edge
     # of a register slot (64, 65), of the register range (1024) and just past
     # it (1025, the memory store), one cluster only, minimum samples above

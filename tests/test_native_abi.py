@@ -75,6 +75,13 @@ def test_create_rejects_bad_params_before_device(native, bad, code):
     assert e.value.code == code
 
 
+def test_create_rejects_more_samples_than_the_device_takes(native):
+    import numpy as np
+    with pytest.raises(native.KincError) as e:
+        native.Engine(np.zeros((2, 32769), np.float32), native.make_params())
+    assert e.value.code == native.KINC_E_UNSUPPORTED
+
+
 @pytest.mark.parametrize("field,value", [("pearson_fp64", 2), ("pearson_fp64", -1),
                                          ("batch_capacity", -1), ("batch_capacity", (1 << 24) + 1)])
 def test_create_rejects_bad_extension_fields(native, field, value):
