@@ -144,14 +144,16 @@ KINC_HD float kinc_expf(float x)
    return (x != x) ? x : r;
 }
 
-KINC_HD float kinc_logf(float x)
+/* The arithmetic of kinc_logf for positive finite x; other x give
+ * unspecified values (NaN stays NaN). */
+KINC_HD float kinc_logf_core(float x)
 {
    const float SQRTHF = 0.707106781186547524f;
    const float C1 = 0.693359375f;
    const float C2 = -2.12194440e-4f;
 
-   /* m in [0.5, 1) for positive finite x; the values computed for zero,
-    * negative, infinite and NaN x are replaced at the end */
+   /* m in [0.5, 1) for positive finite x; kinc_logf replaces the values
+    * computed for zero, negative, infinite and NaN x */
    int e = 0;
    float m = kinc_frexpf(x, &e);
 
@@ -176,12 +178,24 @@ KINC_HD float kinc_logf(float x)
    y = kinc_fmaf(fe, C2, y);
    y = kinc_fmaf(-0.5f, z, y);
    float r = f + y;
-   r = kinc_fmaf(fe, C1, r);
+   return kinc_fmaf(fe, C1, r);
+}
 
+KINC_HD float kinc_logf(float x)
+{
+   const float r = kinc_logf_core(x);
    /* log(+inf) = +inf, log(+-0) = -inf, log(x < 0) = log(NaN) = NaN */
    const int regular = (x > 0.0f) && (x < INFINITY);
    const float special = (x == 0.0f) ? -INFINITY : ((x > 0.0f) ? x : NAN);
    return regular ? r : special;
+}
+
+/* kinc_logf for finite x >= 1 or NaN, the sums of a log-sum-exp whose
+ * largest term is exp(0) = 1: the special-value cases never apply, so they
+ * are left out.  The same bits as kinc_logf for finite x >= 1, NaN for NaN. */
+KINC_HD float kinc_logf_ge1(float x)
+{
+   return kinc_logf_core(x);
 }
 
 #endif

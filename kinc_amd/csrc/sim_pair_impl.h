@@ -640,7 +640,7 @@ __device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, 
          float sum = 0.0f;
 #pragma unroll
          for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf((B[k].w + lgm[k]) - mx);
-         const float lp = mx + kinc_logf(sum);
+         const float lp = mx + kinc_logf_ge1(sum);   /* sum >= 1 or NaN */
          st.LPX(m) = lp;
          const bool valid = lane_id() + m * WAVE < st.n;
 #pragma unroll
@@ -696,7 +696,7 @@ __device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, 
          float sum = 0.0f;
 #pragma unroll
          for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf((logpi[k] + lgm[k]) - mx);
-         float lp = mx + kinc_logf(sum);
+         float lp = mx + kinc_logf_ge1(sum);   /* sum >= 1 or NaN */
          st.LPX(m) = lp;
 #pragma unroll
          for ( int k = 0; k < K; ++k ) st.LG(k, m) = lgm[k] + (-lp);

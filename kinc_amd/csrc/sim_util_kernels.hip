@@ -255,6 +255,7 @@ __global__ void k_eval_math(int fn, const float* in, float* out, int64_t n)
    case 0: r = kinc_expf(x); break;
    case 1: r = kinc_logf(x); break;
    case 2: r = sqrtf(x); break;
+   case 4: r = kinc_logf_ge1(x); break;
    default: r = 1.0f / x; break;
    }
    out[i] = r;
@@ -262,7 +263,7 @@ __global__ void k_eval_math(int fn, const float* in, float* out, int64_t n)
 
 extern "C" int kinc_selftest_math(int32_t fn, const float* in, float* out, int64_t n)
 {
-   if ( fn < 0 || fn > 3 || n < 0 || (n > 0 && (!in || !out)) ) return -1;
+   if ( fn < 0 || fn > 4 || n < 0 || (n > 0 && (!in || !out)) ) return -1;
    if ( n == 0 ) return 0;
    float* d_in = nullptr;
    float* d_out = nullptr;

@@ -30,3 +30,21 @@ def test_device_math_bit_exact(fn, name):
     bad = np.nonzero(~same)[0]
     assert bad.size == 0, "%s differs at %d inputs, e.g. x=%r dev=%r host=%r" % (
         name, bad.size, x[bad[:3]], dev[bad[:3]], host[bad[:3]])
+
+
+def test_device_log_ge1_equals_log():
+    """The E step's log of a log-sum-exp sum (at least 1, or NaN) leaves out
+    the special-value cases; for such inputs it must give the general log's
+    bits (NaN for NaN)."""
+    rng = np.random.default_rng(1)
+    x = np.concatenate([
+        np.linspace(1.0, 8.0, 3_000_001, dtype=np.float32),
+        rng.uniform(1.0, 8.0, 1_000_000).astype(np.float32),
+        np.nextafter(np.float32(1.0), np.float32(2.0)) + np.zeros(1, np.float32),
+        np.array([1.0, 2.0, 4.0, 8.0, np.nan], np.float32)])
+    dev = _native.device_math(4, x)
+    host = oracle.math_array(1, x)
+    same = (dev.view(np.uint32) == host.view(np.uint32)) | (np.isnan(dev) & np.isnan(host))
+    bad = np.nonzero(~same)[0]
+    assert bad.size == 0, "log_ge1 differs at %d inputs, e.g. x=%r dev=%r host=%r" % (
+        bad.size, x[bad[:3]], dev[bad[:3]], host[bad[:3]])
