@@ -17,8 +17,9 @@ extern "C" {
 #endif
 
 /* out[i] = f(in[i]) on the current device for f = 0 exp, 1 log, 2 sqrt,
- * 3 reciprocal (1/x), 4 log for finite x >= 1 or NaN (the log-sum-exp form
- * the E step uses).  Host buffers.  Returns 0 on success, < 0 on error. */
+ * 3 reciprocal (1/x), 4 log for finite x >= 1 or NaN, 5 exp for x <= 0 or
+ * NaN (the log-sum-exp forms the E step uses).  Host buffers.  Returns 0 on
+ * success, < 0 on error. */
 int kinc_selftest_math(int32_t fn, const float* in, float* out, int64_t n);
 
 #ifdef __cplusplus

@@ -226,8 +226,8 @@ class Engine:
 
 
 def device_math(fn, x):
-    """Evaluate exp (0), log (1), sqrt (2), 1/x (3) or the log for finite
-    x >= 1 or NaN (4) on the device."""
+    """Evaluate exp (0), log (1), sqrt (2), 1/x (3), the log for finite
+    x >= 1 or NaN (4) or the exp for x <= 0 or NaN (5) on the device."""
     x = np.ascontiguousarray(x, np.float32)
     out = np.empty_like(x)
     rc = lib().kinc_selftest_math(fn, x.ctypes.data, out.ctypes.data, x.size)

@@ -118,16 +118,12 @@ KINC_HD float kinc_clampf(float x, float lo, float hi)
 #endif
 }
 
-KINC_HD float kinc_expf(float x)
+/* The arithmetic of kinc_expf after the argument is clamped. */
+KINC_HD float kinc_expf_reduced(float xc)
 {
-   /* Outside [-104, 89] the result is 0 or +inf; clamping there lets the
-    * final ldexp produce them with no extra case (exp(-104) < 2^-150 rounds
-    * to 0, exp(89) overflows). */
    const float LOG2EF = 1.44269504088896341f;
    const float C1 = 0.693359375f;
    const float C2 = -2.12194440e-4f;
-   float xc = kinc_clampf(x, -104.0f, 89.0f);
-
    float nf = kinc_rintf(xc * LOG2EF);
    float g = kinc_fmaf(-nf, C1, xc);
    g = kinc_fmaf(-nf, C2, g);
@@ -139,9 +135,30 @@ KINC_HD float kinc_expf(float x)
    q = kinc_fmaf(q, g, 4.999999403953552e-1f);
    q = kinc_fmaf(q, g, 1.0f);
    float p = kinc_fmaf(q, g, 1.0f);
+   return kinc_ldexpf(p, (int)nf);    /* -150 <= nf <= 128 */
+}
 
-   float r = kinc_ldexpf(p, (int)nf);    /* -150 <= nf <= 128 */
+KINC_HD float kinc_expf(float x)
+{
+   /* Outside [-104, 89] the result is 0 or +inf; clamping there lets the
+    * final ldexp produce them with no extra case (exp(-104) < 2^-150 rounds
+    * to 0, exp(89) overflows). */
+   const float r = kinc_expf_reduced(kinc_clampf(x, -104.0f, 89.0f));
    return (x != x) ? x : r;
+}
+
+/* kinc_expf for x <= 0 or NaN, the arguments v - max(v) of a log-sum-exp.
+ * On the device: the lower clamp alone, as a select that a NaN passes
+ * through; the NaN then stays NaN through the reduction (the device's
+ * float-to-int conversion maps it to 0, and ldexp keeps it).  The same value
+ * as kinc_expf for such x, which the host evaluates directly. */
+KINC_HD float kinc_expf_le0(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   return kinc_expf_reduced(x < -104.0f ? -104.0f : x);
+#else
+   return kinc_expf(x);
+#endif
 }
 
 /* The arithmetic of kinc_logf for positive finite x; other x give

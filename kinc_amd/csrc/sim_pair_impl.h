@@ -639,7 +639,7 @@ __device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, 
          for ( int k = 0; k < K; ++k ) mx = fmaxf(mx, B[k].w + lgm[k]);
          float sum = 0.0f;
 #pragma unroll
-         for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf((B[k].w + lgm[k]) - mx);
+         for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf_le0((B[k].w + lgm[k]) - mx);
          const float lp = mx + kinc_logf_ge1(sum);   /* sum >= 1 or NaN */
          st.LPX(m) = lp;
          const bool valid = lane_id() + m * WAVE < st.n;
@@ -695,7 +695,7 @@ __device__ __forceinline__ void gmm_estep(int nslots, ST& st, const float* prm, 
          }
          float sum = 0.0f;
 #pragma unroll
-         for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf((logpi[k] + lgm[k]) - mx);
+         for ( int k = 0; k < K; ++k ) sum = sum + kinc_expf_le0((logpi[k] + lgm[k]) - mx);
          float lp = mx + kinc_logf_ge1(sum);   /* sum >= 1 or NaN */
          st.LPX(m) = lp;
 #pragma unroll
@@ -815,7 +815,8 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
                float l;
                if constexpr ( ST::PMAX == 0 ) l = gmm_logpdf(xa, xb, PA[k], PB[k]) + (-lp);
                else l = st.LG(k, m);
-               tile[(1 + k) * STR + lane] = kinc_expf(l - gmax[k]);
+               /* l <= gmax[k] on the valid points (gmax is their maximum) */
+               tile[(1 + k) * STR + lane] = kinc_expf_le0(l - gmax[k]);
                const float gm = kinc_expf(l);
                /* the memory store keeps gamma only where the labels of a
                 * 100-iteration fit will read it; pass B recomputes it */

@@ -256,6 +256,7 @@ __global__ void k_eval_math(int fn, const float* in, float* out, int64_t n)
    case 1: r = kinc_logf(x); break;
    case 2: r = sqrtf(x); break;
    case 4: r = kinc_logf_ge1(x); break;
+   case 5: r = kinc_expf_le0(x); break;
    default: r = 1.0f / x; break;
    }
    out[i] = r;
@@ -263,7 +264,7 @@ __global__ void k_eval_math(int fn, const float* in, float* out, int64_t n)
 
 extern "C" int kinc_selftest_math(int32_t fn, const float* in, float* out, int64_t n)
 {
-   if ( fn < 0 || fn > 4 || n < 0 || (n > 0 && (!in || !out)) ) return -1;
+   if ( fn < 0 || fn > 5 || n < 0 || (n > 0 && (!in || !out)) ) return -1;
    if ( n == 0 ) return 0;
    float* d_in = nullptr;
    float* d_out = nullptr;

@@ -48,3 +48,20 @@ def test_device_log_ge1_equals_log():
     bad = np.nonzero(~same)[0]
     assert bad.size == 0, "log_ge1 differs at %d inputs, e.g. x=%r dev=%r host=%r" % (
         bad.size, x[bad[:3]], dev[bad[:3]], host[bad[:3]])
+
+
+def test_device_exp_le0_equals_exp():
+    """The E step's exponentials of v - max(v) (at most 0, or NaN) clamp from
+    below only; for such inputs they must give the general exp's bits."""
+    rng = np.random.default_rng(2)
+    x = np.concatenate([
+        -np.linspace(0.0, 120.0, 3_000_001, dtype=np.float32),
+        rng.uniform(-110.0, 0.0, 1_000_000).astype(np.float32),
+        -(np.arange(1, 1 << 16, dtype=np.uint32)).view(np.float32),   # tiny negatives
+        np.array([0.0, -0.0, -103.9, -104.0, -104.1, -87.33655, -np.inf, np.nan], np.float32)])
+    dev = _native.device_math(5, x)
+    host = oracle.math_array(0, x)
+    same = (dev.view(np.uint32) == host.view(np.uint32)) | (np.isnan(dev) & np.isnan(host))
+    bad = np.nonzero(~same)[0]
+    assert bad.size == 0, "exp_le0 differs at %d inputs, e.g. x=%r dev=%r host=%r" % (
+        bad.size, x[bad[:3]], dev[bad[:3]], host[bad[:3]])
