@@ -845,7 +845,9 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       const float gsum = __shfl(acc, 1 + lane);
       const float mxs = __shfl(acc, 1 + K + 2 * lane);
       const float mys = __shfl(acc, 2 + K + 2 * lane);
-      const float logGamma = lane_pick<K>(gmax) + kinc_logf(gsum);
+      /* gsum >= 1 (its largest term is exp(0)) or NaN; likewise sum below,
+       * and v - mx <= 0 or NaN on lanes k < K */
+      const float logGamma = lane_pick<K>(gmax) + kinc_logf_ge1(gsum);
       const float lpk = prm[8 * (lane < K ? lane : 0) + PRM_LPI];
       const float v = lpk + logGamma;
       float mx = -INFINITY;
@@ -857,10 +859,10 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       }
       float sum = 0.0f;
       /* exp(v_k - mx) on lane k, all at once; summed in k order */
-      const float ek = kinc_expf(v - mx);
+      const float ek = kinc_expf_le0(v - mx);
 #pragma unroll
       for ( int k = 0; k < K; ++k ) sum = sum + bcast(ek, k);
-      const float logGammaSum = mx + kinc_logf(sum);
+      const float logGammaSum = mx + kinc_logf_ge1(sum);
 
       const float Gam = kinc_expf(logGamma);
       const float cm = 1.0f / Gam;
