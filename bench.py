@@ -55,11 +55,11 @@ WORKLOADS = {
 # HBM bytes per launch of the dominant pair kernel, from the FETCH_SIZE and
 # WRITE_SIZE passes (scaled by a copy kernel of known size) recorded in:
 TRAFFIC_SOURCE = "profiles/k_pairs_gmm_spearman_20000x500_counters.md"
-TRAFFIC_BYTES_PER_LAUNCH = 1508.8e6
+TRAFFIC_BYTES_PER_LAUNCH = 1499.8e6
 # VALU wave64 instructions per second of the same kernel (SQ_INSTS_VALU per
 # dispatch / duration, same record) against the issue ceiling: 1024 SIMDs x
 # 2.4 GHz / 4 cycles per wave64 instruction
-VALU_ISSUE_RATE = 7.324e11
+VALU_ISSUE_RATE = 7.396e11
 VALU_ISSUE_CEILING = 1024 * 2.4e9 / 4
 BLOCK = 32768
 
