@@ -455,7 +455,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; the pair space would shard "
+        print("bench.py: measures one GPU only; the pair space would shard "
               "across GPUs by contiguous work-block ranges (see the sharding notes); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
