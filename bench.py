@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "gene pairs/sec GMM+Spearman 20k×500, batched vs plain; p50/p99 query latency"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+CLOCK_HZ = 2.4e9   # MI355X peak engine clock
 WORKLOADS = {
     "H": "KINC similarity, GMM (K=1..5, BIC) + Spearman, pre/post outliers, minsamp 30; "
          "20000 genes x 500 samples FP32, module generator, 10% NaN",
@@ -410,6 +411,35 @@ def rmt_leg(gemgen, with_cpu):
     return out
 
 
+def valu_calibration():
+    """Issue rate of v_add_f32 streams with 16 one-wave workgroups per CU
+    (4 waves per SIMD, as the pair kernels run): wave64 instructions per
+    second and cycles per instruction per SIMD at the 2.4 GHz peak clock,
+    for full, partial and dependent streams."""
+    import torch
+    from kinc_amd import _native as native
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    wgs = cus * 16
+    lds = 160 * 1024 // 16
+    iters = 20000
+    rows = []
+    for mode, (chains, lanes) in enumerate(native.VALU_PROBE_MODES):
+        ms = native.valu_probe(mode, iters, wgs, lds)
+        insts = 128.0 * iters * wgs
+        rate = insts / (ms * 1e-3)
+        cpi = (cus * 4) * CLOCK_HZ / rate
+        rows.append({"mode": mode, "chains": chains, "lanes": lanes, "ms": ms,
+                     "wave_insts": insts, "insts_per_s": rate,
+                     "cycles_per_inst_per_simd": cpi})
+        log("valu probe mode %d (%d chains, %d lanes): %.3f ms, %.3g wave insts/s, "
+            "%.2f cycles per instruction per SIMD" % (mode, chains, lanes, ms, rate, cpi))
+    print(json.dumps({"valu_calibration": rows, "cus": cus, "workgroups": wgs,
+                      "iters": iters, "clock_hz": CLOCK_HZ}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -438,6 +468,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=0.5)
     ap.add_argument("--serve-max-genes", type=int, default=16)
     ap.add_argument("--serve-big-query", type=int, default=100)
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only run the VALU issue-rate probe (v_add_f32 streams, 4 waves/SIMD)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -452,6 +484,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}))
+        return
+
+    if args.valu_calibration:
+        valu_calibration()
         return
 
     if args.gpus != 1:

@@ -69,6 +69,8 @@ SIGNATURES = [
     ("kinc_sim_set_profiling", ctypes.c_int, [_P, ctypes.c_int32]),
     ("kinc_sim_profile_read", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int32]),
     ("kinc_selftest_math", ctypes.c_int, [ctypes.c_int32, _P, _P, ctypes.c_int64]),
+    ("kinc_selftest_valu_probe", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
     ("kinc_pair_count", ctypes.c_int64, [ctypes.c_int32]),
     ("kinc_pair_index", None, [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
                                ctypes.POINTER(ctypes.c_int32)]),
@@ -234,3 +236,17 @@ def device_math(fn, x):
     if rc != KINC_OK:
         raise KincError(rc, "kinc_selftest_math failed")
     return out
+
+
+# (independent chains, active lanes) of each kinc_selftest_valu_probe mode
+VALU_PROBE_MODES = [(8, 64), (8, 16), (8, 4), (1, 64), (1, 16), (8, 32)]
+
+
+def valu_probe(mode, iters, workgroups, lds_bytes):
+    """Duration in ms of one probe launch issuing 128 * iters v_add_f32 per
+    wave on the lanes and chains of `mode` (kinc_selftest_valu_probe)."""
+    ms = ctypes.c_float()
+    rc = lib().kinc_selftest_valu_probe(mode, iters, workgroups, lds_bytes, ctypes.byref(ms))
+    if rc != KINC_OK:
+        raise KincError(rc, "kinc_selftest_valu_probe failed")
+    return float(ms.value)

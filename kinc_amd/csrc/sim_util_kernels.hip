@@ -288,3 +288,84 @@ extern "C" int kinc_selftest_math(int32_t fn, const float* in, float* out, int64
    (void)hipFree(d_out);
    return rc;
 }
+
+/*
+ * VALU issue-rate probe (diagnostic).  Each wave runs `iters` rounds of 128
+ * v_add_f32 on its first LANES lanes, spread over CHAINS independent
+ * accumulators (CHAINS = 1: one dependent chain).  The issue count per wave
+ * is exactly 128 * iters VALU instructions (the loop counter is scalar), so
+ * the launch time gives the device's wave64 VALU issue rate for that exec
+ * mask and dependency pattern, the ceiling the pair kernels are measured
+ * against.
+ */
+template <int CHAINS, int LANES>
+__global__ __launch_bounds__(64) void k_valu_probe(float* out, int iters)
+{
+   const int lane = (int)threadIdx.x;
+   float acc[CHAINS];
+#pragma unroll
+   for ( int c = 0; c < CHAINS; ++c ) acc[c] = (float)c;
+   const float inc = 1.0e-7f * (float)(lane + 1);
+   if ( lane < LANES )
+   {
+      for ( int it = 0; it < iters; ++it )
+      {
+#pragma unroll
+         for ( int r = 0; r < 128 / CHAINS; ++r )
+         {
+#pragma unroll
+            for ( int c = 0; c < CHAINS; ++c )
+            {
+               asm volatile("v_add_f32 %0, %0, %1" : "+v"(acc[c]) : "v"(inc));
+            }
+         }
+      }
+   }
+   float s = 0.0f;
+#pragma unroll
+   for ( int c = 0; c < CHAINS; ++c ) s = s + acc[c];
+   out[(size_t)blockIdx.x * 64 + lane] = s;
+}
+
+/* modes of kinc_selftest_valu_probe: (independent chains, active lanes) */
+static const int kProbeModes[][2] = {{8, 64}, {8, 16}, {8, 4}, {1, 64}, {1, 16}, {8, 32}};
+
+extern "C" int kinc_selftest_valu_probe(int32_t mode, int32_t iters, int32_t workgroups,
+                                        int32_t lds_bytes, float* ms)
+{
+   if ( mode < 0 || mode >= (int)(sizeof(kProbeModes) / sizeof(kProbeModes[0])) || iters < 1 ||
+        workgroups < 1 || lds_bytes < 0 || lds_bytes > 65536 || !ms )
+   {
+      return -1;
+   }
+   const void* fn = nullptr;
+   switch ( mode )
+   {
+   case 0: fn = (const void*)&k_valu_probe<8, 64>; break;
+   case 1: fn = (const void*)&k_valu_probe<8, 16>; break;
+   case 2: fn = (const void*)&k_valu_probe<8, 4>; break;
+   case 3: fn = (const void*)&k_valu_probe<1, 64>; break;
+   case 4: fn = (const void*)&k_valu_probe<1, 16>; break;
+   default: fn = (const void*)&k_valu_probe<8, 32>; break;
+   }
+   float* d_out = nullptr;
+   if ( hipMalloc(&d_out, sizeof(float) * 64 * (size_t)workgroups) != hipSuccess ) return -3;
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   int rc = 0;
+   if ( hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ) rc = -2;
+   for ( int rep = 0; rep < 2 && rc == 0; ++rep )
+   {
+      /* the first launch warms up; the second is timed */
+      void* args[] = {&d_out, &iters};
+      if ( hipEventRecord(e0, 0) != hipSuccess ) rc = -2;
+      if ( rc == 0 && hipLaunchKernel(fn, dim3((unsigned)workgroups), dim3(64), args,
+                                      (size_t)lds_bytes, 0) != hipSuccess ) rc = -2;
+      if ( rc == 0 && hipEventRecord(e1, 0) != hipSuccess ) rc = -2;
+      if ( rc == 0 && hipEventSynchronize(e1) != hipSuccess ) rc = -2;
+      if ( rc == 0 && hipEventElapsedTime(ms, e0, e1) != hipSuccess ) rc = -2;
+   }
+   if ( e0 ) (void)hipEventDestroy(e0);
+   if ( e1 ) (void)hipEventDestroy(e1);
+   (void)hipFree(d_out);
+   return rc;
+}

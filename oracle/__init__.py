@@ -35,6 +35,7 @@ def lib():
             "oracle_logf": (f32, [f32]),
             "oracle_bic": (f32, [ctypes.c_int, f32, ctypes.c_int, ctypes.c_int]),
             "oracle_icl": (f32, [ctypes.c_int, f32, ctypes.c_int, ctypes.c_int, f32]),
+            "oracle_set_plant": (None, [ctypes.c_int]),
             "oracle_gmm_fit_flat": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P, P]),
             "oracle_kmeans_fit_flat": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P]),
             "oracle_mark_outliers_flat": (None, [P, ctypes.c_int, ctypes.c_int, P, ctypes.c_int,
@@ -170,6 +171,17 @@ def bitonic_sort(key, payload):
 
 def bic(K, logL, N, D=2):
     return lib().oracle_bic(K, logL, N, D)
+
+
+def icl(K, logL, N, E, D=2):
+    return lib().oracle_icl(K, logL, N, D, E)
+
+
+def set_plant(which):
+    """Switch a deliberate slip into the oracle's GMM entropy (0: none; 1:
+    taken on gamma instead of log gamma after a converged fit; 2: log pi
+    added to the log responsibilities).  Test hook only."""
+    lib().oracle_set_plant(which)
 
 
 def math_array(fn, x):

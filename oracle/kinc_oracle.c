@@ -212,6 +212,15 @@ static uint16_t pack_iters(const fit_stats* st)
    return v;
 }
 
+/* Test hook: which deliberate slip the entropy takes (0: none).  Only the
+ * tests that check the entropy pin set it. */
+static int g_plant = 0;
+
+void oracle_set_plant(int which)
+{
+   g_plant = which;
+}
+
 /* src/core/pairwise_gmm.cpp:367-445 with the per-fit LCG.
  * labels: N entries written on success.  Returns 1 on success. */
 int oracle_gmm_fit(const vec2* X, int N, int K, int8_t* labels,
@@ -410,12 +419,17 @@ int oracle_gmm_fit(const vec2* X, int N, int K, int8_t* labels,
       }
 
       /* calcEntropy (pairwise_gmm.cpp:348-360).  A label of -1 makes the
-       * reference read before the array (undefined); it is taken as NaN. */
+       * reference read before the array (undefined); it is taken as NaN.
+       * g_plant switches in a deliberate slip for the tests that show the
+       * entropy pin notices one; it is 0 everywhere else. */
       float E = 0;
       for ( int i = 0; i < N; ++i )
       {
          int k = labels[i];
-         E += (k >= 0) ? kinc_logf(loggamma[k * N + i]) : NAN;
+         float v = (k >= 0) ? loggamma[k * N + i] : NAN;
+         if ( g_plant == 1 && local.converged ) v = kinc_expf(v);   /* on gamma */
+         if ( g_plant == 2 && k >= 0 ) v = v + logpi[k];            /* with log pi */
+         E += (k >= 0) ? kinc_logf(v) : NAN;
       }
       *entropy_out = E;
    }
