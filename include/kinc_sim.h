@@ -149,7 +149,10 @@ int kinc_sim_execute_pairs(kinc_sim* h, const int32_t* pairs, const int32_t* seg
                            uint16_t* iters);
 
 /* Same with device buffers (pairs as int32 x,y interleaved), asynchronous on
- * `stream` like kinc_sim_execute_device. */
+ * `stream` like kinc_sim_execute_device.  The device lists are not checked
+ * on the host: a pair outside x > y >= 0, x < G reads gene 0 twice and its
+ * results are meaningless, and a seg_lo[q] outside [0, q] is taken as the
+ * nearest end of that range; neither reads outside the handle's buffers. */
 int kinc_sim_execute_pairs_device(kinc_sim* h, const int32_t* pairs, const int32_t* seg_lo,
                                   int64_t count, int8_t* K, int8_t* labels, float* corr,
                                   uint16_t* iters, void* stream);

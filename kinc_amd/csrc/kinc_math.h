@@ -118,6 +118,22 @@ KINC_HD float kinc_clampf(float x, float lo, float hi)
 #endif
 }
 
+/* The integer value of v, an integral float in [-150, 128], or 0 when v is
+ * NaN.  On the device this is the hardware conversion v_cvt_i32_f32, whose
+ * NaN result (0) the ISA defines; it is written as an instruction because a
+ * C++ float-to-int conversion of NaN is undefined (poison to the compiler),
+ * and kinc_expf_le0 lets NaN reach here.  Not a memory access. */
+KINC_HD int kinc_f2i(float v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   int r;
+   asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+   return r;
+#else
+   return (v != v) ? 0 : (int)v;
+#endif
+}
+
 /* The arithmetic of kinc_expf after the argument is clamped. */
 KINC_HD float kinc_expf_reduced(float xc)
 {
@@ -135,7 +151,7 @@ KINC_HD float kinc_expf_reduced(float xc)
    q = kinc_fmaf(q, g, 4.999999403953552e-1f);
    q = kinc_fmaf(q, g, 1.0f);
    float p = kinc_fmaf(q, g, 1.0f);
-   return kinc_ldexpf(p, (int)nf);    /* -150 <= nf <= 128 */
+   return kinc_ldexpf(p, kinc_f2i(nf));    /* -150 <= nf <= 128, or NaN */
 }
 
 KINC_HD float kinc_expf(float x)
@@ -149,8 +165,8 @@ KINC_HD float kinc_expf(float x)
 
 /* kinc_expf for x <= 0 or NaN, the arguments v - max(v) of a log-sum-exp.
  * On the device: the lower clamp alone, as a select that a NaN passes
- * through; the NaN then stays NaN through the reduction (the device's
- * float-to-int conversion maps it to 0, and ldexp keeps it).  The same value
+ * through; the NaN then stays NaN through the reduction (kinc_f2i maps it
+ * to 0, and ldexp keeps the NaN).  The same value
  * as kinc_expf for such x, which the host evaluates directly. */
 KINC_HD float kinc_expf_le0(float x)
 {

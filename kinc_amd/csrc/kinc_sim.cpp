@@ -51,29 +51,30 @@ namespace {
  * per cluster bound KMAX. */
 const int kSlotList5[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16};
 const int kSlotList8[] = {2, 4, 8};
-const int kMaxBins = 12;
+const int kMaxBins = KINC_MAX_BINS;
 
-thread_local std::string g_create_error;
+thread_local char g_create_error[256];
 
 /* Slot classes used for a handle: every instance up to the one S needs,
  * or the memory-store instance (P = 0) alone above the register range
- * (1,024 samples for up to 5 clusters, 512 for up to 8). */
-std::vector<int> slot_classes(int S, int kmax)
+ * (1,024 samples for up to 5 clusters, 512 for up to 8).  Writes at most
+ * kMaxBins classes to out and returns their number. */
+int slot_classes(int S, int kmax, int* out)
 {
-   std::vector<int> out;
    const int* list = kmax == 5 ? kSlotList5 : kSlotList8;
    const int n = kmax == 5 ? 12 : 3;
    if ( S > 64 * list[n - 1] )
    {
-      out.push_back(0);
-      return out;
+      out[0] = 0;
+      return 1;
    }
+   int c = 0;
    for ( int i = 0; i < n; ++i )
    {
-      out.push_back(list[i]);
+      out[c++] = list[i];
       if ( 64 * list[i] >= S ) break;
    }
-   return out;
+   return c;
 }
 
 /* The pair kernel for (P, KMAX); one instance serves both forms.  With w5,
@@ -125,9 +126,9 @@ struct kinc_sim
    int32_t* d_prev = nullptr;      /* with d_xbuf: previous-greater link per pair */
    int32_t* d_counters = nullptr;  /* [2*kMaxBins]: bin sizes, then pop heads */
    int32_t* d_lists = nullptr;     /* [kMaxBins][KINC_WORK_BLOCK_SIZE] */
-   int32_t* d_binP = nullptr;
-   std::vector<int> classes;       /* slot count of each bin (0 = memory store) */
+   int classes[kMaxBins] = {};     /* slot count of each bin (0 = memory store) */
    int nbins = 0;
+   BinBounds bins{};               /* the same, as the binning kernel takes it */
    char* d_scratch = nullptr;      /* memory-store slices */
    int scratch_slots = 0;
    int cap = KINC_WORK_BLOCK_SIZE; /* largest count of one call */
@@ -145,7 +146,7 @@ struct kinc_sim
    hipEvent_t drained[kAux] = {};
    SimArgs base{};
    size_t lds_bytes = 0;
-   std::string err;
+   char err[256] = {};
    /* kernel timing */
    bool profiling = false;
    /* the per-wave LDS leaves room for 20 waves per CU: small instances run
@@ -158,8 +159,24 @@ struct kinc_sim
 
 static int fail(kinc_sim* h, int code, const std::string& msg)
 {
-   if ( h ) h->err = msg;
-   else g_create_error = msg;
+   char* dst = h ? h->err : g_create_error;
+   std::snprintf(dst, 256, "%s", msg.c_str());
+   return code;
+}
+
+/* a failed HIP call (formats into the handle's buffer; builds no string) */
+static int fail_hip(kinc_sim* h, const char* call, hipError_t e)
+{
+   char* dst = h ? h->err : g_create_error;
+   std::snprintf(dst, 256, "%s: %s", call, hipGetErrorString(e));
+   return KINC_E_DEVICE;
+}
+
+/* fail() for a message that needs no formatting (builds no string) */
+static int fail_c(kinc_sim* h, int code, const char* msg)
+{
+   char* dst = h ? h->err : g_create_error;
+   std::snprintf(dst, 256, "%s", msg);
    return code;
 }
 
@@ -169,7 +186,7 @@ static int fail(kinc_sim* h, int code, const std::string& msg)
       hipError_t e_ = (call);                                                         \
       if ( e_ != hipSuccess )                                                         \
       {                                                                               \
-         return fail((h), KINC_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+         return fail_hip((h), #call, e_);                                              \
       }                                                                               \
    } while ( 0 )
 
@@ -205,123 +222,73 @@ extern "C" void kinc_pair_index(int64_t index, int32_t* x, int32_t* y)
    *y = (int32_t)(index - xx * (xx - 1) / 2);
 }
 
-static int validate(const kinc_sim_params* p, int G, int S, std::string& why)
+static int validate(const kinc_sim_params* p, int G, int S, const char** why)
 {
-   if ( p->clus_method < 0 || p->clus_method > 2 ) { why = "clus_method must be 0 (none), 1 (gmm) or 2 (kmeans)"; return KINC_E_INVALID; }
-   if ( p->corr_method < 0 || p->corr_method > 1 ) { why = "corr_method must be 0 (pearson) or 1 (spearman)"; return KINC_E_INVALID; }
-   if ( p->criterion < 0 || p->criterion > 1 ) { why = "criterion must be 0 (BIC) or 1 (ICL)"; return KINC_E_INVALID; }
-   if ( p->min_samples < 1 ) { why = "min_samples must be >= 1"; return KINC_E_INVALID; }
+   if ( p->clus_method < 0 || p->clus_method > 2 ) { *why = "clus_method must be 0 (none), 1 (gmm) or 2 (kmeans)"; return KINC_E_INVALID; }
+   if ( p->corr_method < 0 || p->corr_method > 1 ) { *why = "corr_method must be 0 (pearson) or 1 (spearman)"; return KINC_E_INVALID; }
+   if ( p->criterion < 0 || p->criterion > 1 ) { *why = "criterion must be 0 (BIC) or 1 (ICL)"; return KINC_E_INVALID; }
+   if ( p->min_samples < 1 ) { *why = "min_samples must be >= 1"; return KINC_E_INVALID; }
    if ( p->min_clusters < 1 || p->max_clusters < 1 || p->max_clusters > KINC_MAX_CLUSTER_SIZE )
    {
-      why = "cluster range must lie in [1, 64]";
+      *why = "cluster range must lie in [1, 64]";
       return KINC_E_INVALID;
    }
    if ( p->max_clusters < p->min_clusters )
    {
       /* similarity.cpp:181-187 */
-      why = "Minimum clusters must be less than or equal to maximum clusters.";
+      *why = "Minimum clusters must be less than or equal to maximum clusters.";
       return KINC_E_INVALID;
    }
-   if ( p->ksize < 1 ) { why = "ksize must be >= 1"; return KINC_E_INVALID; }
+   if ( p->ksize < 1 ) { *why = "ksize must be >= 1"; return KINC_E_INVALID; }
    if ( p->batch_capacity < 0 || p->batch_capacity > (1 << 24) )
    {
-      why = "batch_capacity must lie in [0, 2^24]";
+      *why = "batch_capacity must lie in [0, 2^24]";
       return KINC_E_INVALID;
    }
-   if ( p->form != KINC_FORM_PLAIN && p->form != KINC_FORM_PERSISTENT ) { why = "form must be 0 (plain) or 1 (persistent)"; return KINC_E_INVALID; }
-   if ( p->pearson_fp64 != 0 && p->pearson_fp64 != 1 ) { why = "pearson_fp64 must be 0 or 1"; return KINC_E_INVALID; }
+   if ( p->form != KINC_FORM_PLAIN && p->form != KINC_FORM_PERSISTENT ) { *why = "form must be 0 (plain) or 1 (persistent)"; return KINC_E_INVALID; }
+   if ( p->pearson_fp64 != 0 && p->pearson_fp64 != 1 ) { *why = "pearson_fp64 must be 0 or 1"; return KINC_E_INVALID; }
    for ( int r : p->reserved )
    {
-      if ( r != 0 ) { why = "reserved fields must be zero"; return KINC_E_INVALID; }
+      if ( r != 0 ) { *why = "reserved fields must be zero"; return KINC_E_INVALID; }
    }
-   if ( G < 2 || S < 1 ) { why = "need at least 2 genes and 1 sample"; return KINC_E_INVALID; }
+   if ( G < 2 || S < 1 ) { *why = "need at least 2 genes and 1 sample"; return KINC_E_INVALID; }
    if ( p->max_clusters > 8 )
    {
-      why = "max_clusters > 8 is not built on the device path";
+      *why = "max_clusters > 8 is not built on the device path";
       return KINC_E_UNSUPPORTED;
    }
    if ( S > KINC_MAX_SAMPLES )
    {
-      why = "sample_size > " + std::to_string(KINC_MAX_SAMPLES) + " is not supported";
+      *why = "sample_size > 32768 is not supported";
       return KINC_E_UNSUPPORTED;
    }
    return KINC_OK;
 }
 
-extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
-                               int32_t G, int32_t S, int32_t device, kinc_sim** out)
+/*
+ * The parameters, bins and working-set carve-up of a handle for an S-sample
+ * matrix: everything but the device buffers.  Allocates nothing (the
+ * allocation-free entry point uses it too).
+ */
+static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int S)
 {
-   if ( !params || !emx || !out ) return fail(nullptr, KINC_E_INVALID, "null argument");
-   *out = nullptr;
-   std::string why;
-   int rc = validate(params, G, S, why);
-   if ( rc != KINC_OK ) return fail(nullptr, rc, why);
-
-   kinc_sim* h = new kinc_sim();
    h->p = *params;
-   h->device = device;
    h->G = G;
    h->S = S;
    h->kmax = params->max_clusters <= 5 ? 5 : 8;
-
-   auto bail = [&](int code) {
-      g_create_error = h->err;
-      kinc_sim_destroy(h);
-      return code;
-   };
-#define CREATE_TRY(call)                                                             \
-   do                                                                                \
-   {                                                                                 \
-      hipError_t e_ = (call);                                                        \
-      if ( e_ != hipSuccess )                                                        \
-      {                                                                              \
-         h->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
-         return bail(KINC_E_DEVICE);                                                 \
-      }                                                                              \
-   } while ( 0 )
-
-   CREATE_TRY(hipSetDevice(device));
-   hipDeviceProp_t prop;
-   CREATE_TRY(hipGetDeviceProperties(&prop, device));
-   h->num_cus = prop.multiProcessorCount;
-   CREATE_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-   for ( int i = 0; i < kinc_sim::kAux; ++i )
-   {
-      CREATE_TRY(hipStreamCreateWithFlags(&h->aux[i], hipStreamNonBlocking));
-      CREATE_TRY(hipEventCreateWithFlags(&h->drained[i], hipEventDisableTiming));
-   }
-   CREATE_TRY(hipEventCreateWithFlags(&h->binned, hipEventDisableTiming));
-
-   const size_t emx_bytes = (size_t)G * (size_t)S * sizeof(float);
-   CREATE_TRY(hipMalloc(&h->d_emx, emx_bytes));
-   CREATE_TRY(hipMemcpy(h->d_emx, emx, emx_bytes, hipMemcpyHostToDevice));
-
-   /* log(N) of the BIC term (pairwise_clustering.cpp:160), evaluated by the
-    * host libm once per possible N */
-   std::vector<double> logN((size_t)S + 1);
-   for ( int n = 0; n <= S; ++n ) logN[(size_t)n] = std::log((double)n);
-   CREATE_TRY(hipMalloc(&h->d_logN, sizeof(double) * logN.size()));
-   CREATE_TRY(hipMemcpy(h->d_logN, logN.data(), sizeof(double) * logN.size(), hipMemcpyHostToDevice));
-
    h->cap = params->batch_capacity > 0 ? params->batch_capacity : KINC_WORK_BLOCK_SIZE;
-   const size_t B = (size_t)h->cap;
-   CREATE_TRY(hipMalloc(&h->d_blockN, sizeof(int32_t) * B));
-   CREATE_TRY(hipMalloc(&h->d_chunkMax, sizeof(int32_t) * (B / 64 + 1)));
-   CREATE_TRY(hipMalloc(&h->d_counters, sizeof(int32_t) * 2 * kMaxBins));
-   CREATE_TRY(hipMalloc(&h->d_lists, sizeof(int32_t) * B * kMaxBins));
 
    /* bins: every register-slot count up to the one S needs, or the single
     * memory-store class; a bin takes pairs with 64*P >= N (the memory-store
     * bin takes every pair) */
-   h->classes = slot_classes(S, h->kmax);
-   h->nbins = (int)h->classes.size();
-   std::vector<int32_t> binP;
-   for ( int P : h->classes ) binP.push_back(P > 0 ? P : (1 << 20));
-   CREATE_TRY(hipMalloc(&h->d_binP, sizeof(int32_t) * binP.size()));
-   CREATE_TRY(hipMemcpy(h->d_binP, binP.data(), sizeof(int32_t) * binP.size(), hipMemcpyHostToDevice));
+   h->nbins = slot_classes(S, h->kmax, h->classes);
+   for ( int b = 0; b < kMaxBins; ++b )
+   {
+      h->bins.P[b] = b < h->nbins ? (h->classes[b] > 0 ? h->classes[b] : (1 << 20)) : 0;
+   }
 
    SimArgs& a = h->base;
-   a.emx = h->d_emx;
+   a = SimArgs{};
    a.G = G;
    a.S = S;
    a.clus = params->clus_method;
@@ -334,24 +301,9 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
    a.pre_out = params->pre_out ? 1 : 0;
    a.post_out = params->post_out ? 1 : 0;
    a.pearson_fp64 = params->pearson_fp64;
-   a.logN = h->d_logN;
-   a.blockN = h->d_blockN;
-   a.chunkMax = h->d_chunkMax;
    a.np2 = next_pow2(S);
    a.tile_stride = KINC_TILE_STRIDE;
    a.mem_slots = (S + 63) / 64;
-   if ( params->corr_method == KINC_CORR_PEARSON &&
-        (h->classes[0] != 0 || params->clus_method == KINC_CLUS_NONE) &&
-        sizeof(float) * 2 * B * (size_t)S <= ((size_t)4 << 30) )
-   {
-      /* Pearson's stale reads copy earlier pairs' compact X from here
-       * instead of fetching and compacting their rows again (256 MiB for
-       * 32768 pairs of 1024 samples; at most 4 GiB) */
-      CREATE_TRY(hipMalloc(&h->d_xbuf, sizeof(float) * 2 * B * (size_t)S));
-      CREATE_TRY(hipMalloc(&h->d_prev, sizeof(int32_t) * B));
-      a.xbuf = h->d_xbuf;
-      a.prev = h->d_prev;
-   }
    const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
    const size_t sort_bytes = sizeof(float) * 2 * (size_t)a.np2;
 
@@ -389,6 +341,88 @@ extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
       a.lds_prm = 0;
       a.lds_tile = (int32_t)align16(sizeof(float) * 8 * 8);
       h->lds_bytes = align16((size_t)a.lds_tile + tile_bytes);
+      h->w5 = false;
+   }
+}
+
+extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
+                               int32_t G, int32_t S, int32_t device, kinc_sim** out)
+{
+   if ( !params || !emx || !out ) return fail(nullptr, KINC_E_INVALID, "null argument");
+   *out = nullptr;
+   const char* why = nullptr;
+   int rc = validate(params, G, S, &why);
+   if ( rc != KINC_OK ) return fail_c(nullptr, rc, why);
+
+   kinc_sim* h = new kinc_sim();
+   h->device = device;
+   setup_layout(h, params, G, S);
+
+   auto bail = [&](int code) {
+      std::snprintf(g_create_error, sizeof(g_create_error), "%s", h->err);
+      kinc_sim_destroy(h);
+      return code;
+   };
+#define CREATE_TRY(call)                                                             \
+   do                                                                                \
+   {                                                                                 \
+      hipError_t e_ = (call);                                                        \
+      if ( e_ != hipSuccess )                                                        \
+      {                                                                              \
+         std::snprintf(h->err, sizeof(h->err), "%s: %s", #call, hipGetErrorString(e_)); \
+         return bail(KINC_E_DEVICE);                                                 \
+      }                                                                              \
+   } while ( 0 )
+
+   CREATE_TRY(hipSetDevice(device));
+   hipDeviceProp_t prop;
+   CREATE_TRY(hipGetDeviceProperties(&prop, device));
+   h->num_cus = prop.multiProcessorCount;
+   CREATE_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+   for ( int i = 0; i < kinc_sim::kAux; ++i )
+   {
+      CREATE_TRY(hipStreamCreateWithFlags(&h->aux[i], hipStreamNonBlocking));
+      CREATE_TRY(hipEventCreateWithFlags(&h->drained[i], hipEventDisableTiming));
+   }
+   CREATE_TRY(hipEventCreateWithFlags(&h->binned, hipEventDisableTiming));
+
+   const size_t emx_bytes = (size_t)G * (size_t)S * sizeof(float);
+   CREATE_TRY(hipMalloc(&h->d_emx, emx_bytes));
+   CREATE_TRY(hipMemcpy(h->d_emx, emx, emx_bytes, hipMemcpyHostToDevice));
+
+   /* log(N) of the BIC term (pairwise_clustering.cpp:160), evaluated by the
+    * host libm once per possible N */
+   std::vector<double> logN((size_t)S + 1);
+   for ( int n = 0; n <= S; ++n ) logN[(size_t)n] = std::log((double)n);
+   CREATE_TRY(hipMalloc(&h->d_logN, sizeof(double) * logN.size()));
+   CREATE_TRY(hipMemcpy(h->d_logN, logN.data(), sizeof(double) * logN.size(), hipMemcpyHostToDevice));
+
+   const size_t B = (size_t)h->cap;
+   CREATE_TRY(hipMalloc(&h->d_blockN, sizeof(int32_t) * B));
+   CREATE_TRY(hipMalloc(&h->d_chunkMax, sizeof(int32_t) * (B / 64 + 1)));
+   CREATE_TRY(hipMalloc(&h->d_counters, sizeof(int32_t) * 2 * kMaxBins));
+   CREATE_TRY(hipMalloc(&h->d_lists, sizeof(int32_t) * B * kMaxBins));
+
+   SimArgs& a = h->base;
+   a.emx = h->d_emx;
+   a.logN = h->d_logN;
+   a.blockN = h->d_blockN;
+   a.chunkMax = h->d_chunkMax;
+   if ( params->corr_method == KINC_CORR_PEARSON &&
+        (h->classes[0] != 0 || params->clus_method == KINC_CLUS_NONE) &&
+        sizeof(float) * 2 * B * (size_t)S <= ((size_t)4 << 30) )
+   {
+      /* Pearson's stale reads copy earlier pairs' compact X from here
+       * instead of fetching and compacting their rows again (256 MiB for
+       * 32768 pairs of 1024 samples; at most 4 GiB) */
+      CREATE_TRY(hipMalloc(&h->d_xbuf, sizeof(float) * 2 * B * (size_t)S));
+      CREATE_TRY(hipMalloc(&h->d_prev, sizeof(int32_t) * B));
+      a.xbuf = h->d_xbuf;
+      a.prev = h->d_prev;
+   }
+   if ( h->classes[0] == 0 )
+   {
+      /* memory store: one scratch slice per resident workgroup */
       const void* fn = kinc_select_pair_kernel(0, h->kmax);
       int per_cu = 0;
       CREATE_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
@@ -551,9 +585,9 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
    if ( h->p.form == KINC_FORM_PLAIN )
    {
       /* the slot count S needs (or the memory store) for every pair */
-      const int P = h->classes.back();
+      const int P = h->classes[h->nbins - 1];
       const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
-      if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
+      if ( !fn ) return fail_c(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       a.list = nullptr;
       for ( int first = 0; first < cnt; first += h->p.ksize )
       {
@@ -583,7 +617,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
        * the same, so resident waves stride over the whole block in one launch */
       const int P = h->classes[0];
       const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
-      if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
+      if ( !fn ) return fail_c(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
       HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
       if ( per_cu < 1 ) per_cu = 1;
@@ -606,24 +640,30 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
    }
    {
       const int32_t* Np = h->d_blockN;
-      const int32_t* bp = h->d_binP;
+      BinBounds bb = h->bins;
       int nb = h->nbins;
       int32_t* ctr = h->d_counters;
       int32_t* lists = h->d_lists;
-      void* args[] = {&Np, (void*)&cnt, &bp, &nb, &ctr, &lists};
+      void* args[] = {&Np, (void*)&cnt, &bb, &nb, &ctr, &lists};
       int rc = launch(h, kinc_kernel_bin_pairs(), (unsigned)((cnt + 255) / 256), 256, args, 0, s);
       if ( rc ) return rc;
    }
    /* Bins run concurrently: a bin with few pairs would otherwise leave most
-    * of the device idle for one pair's latency (milliseconds). */
-   HIP_TRY(h, hipEventRecord(h->binned, s));
-   for ( int i = 0; i < kinc_sim::kAux; ++i ) HIP_TRY(h, hipStreamWaitEvent(h->aux[i], h->binned, 0));
+    * of the device idle for one pair's latency (milliseconds).  Without
+    * auxiliary streams (the allocation-free entry point) they run one after
+    * another on s. */
+   const bool fork = h->aux[0] != nullptr;
+   if ( fork )
+   {
+      HIP_TRY(h, hipEventRecord(h->binned, s));
+      for ( int i = 0; i < kinc_sim::kAux; ++i ) HIP_TRY(h, hipStreamWaitEvent(h->aux[i], h->binned, 0));
+   }
    for ( int b = h->nbins - 1; b >= 0; --b )
    {
       const int P = h->classes[b];
-      hipStream_t sb = h->aux[(h->nbins - 1 - b) % kinc_sim::kAux];
+      hipStream_t sb = fork ? h->aux[(h->nbins - 1 - b) % kinc_sim::kAux] : s;
       const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
-      if ( !fn ) return fail(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
+      if ( !fn ) return fail_c(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
       HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
       if ( per_cu < 1 ) per_cu = 1;
@@ -637,7 +677,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       int rc = launch_pairs(h, P, fn, (unsigned)grid, args, sb);
       if ( rc ) return rc;
    }
-   for ( int i = 0; i < kinc_sim::kAux; ++i )
+   for ( int i = 0; fork && i < kinc_sim::kAux; ++i )
    {
       HIP_TRY(h, hipEventRecord(h->drained[i], h->aux[i]));
       HIP_TRY(h, hipStreamWaitEvent(s, h->drained[i], 0));
@@ -750,7 +790,6 @@ extern "C" void kinc_sim_destroy(kinc_sim* h)
    (void)hipFree(h->d_prev);
    (void)hipFree(h->d_counters);
    (void)hipFree(h->d_lists);
-   (void)hipFree(h->d_binP);
    (void)hipFree(h->d_scratch);
    (void)hipFree(h->d_pairs);
    (void)hipFree(h->d_seg);
@@ -776,5 +815,5 @@ extern "C" void kinc_sim_destroy(kinc_sim* h)
 
 extern "C" const char* kinc_sim_last_error(const kinc_sim* h)
 {
-   return h ? h->err.c_str() : g_create_error.c_str();
+   return h ? h->err : g_create_error;
 }

@@ -24,6 +24,16 @@
  * offset field. */
 #define KINC_TILE_STRIDE 68
 
+/* Bins of the persistent form: at most this many register-slot classes. */
+#define KINC_MAX_BINS 12
+
+/* Slot count of each bin (a pair goes to the first bin with 64*P >= N; the
+ * memory-store bin's entry is 2^20), passed to the binning kernel by value. */
+struct BinBounds
+{
+   int32_t P[KINC_MAX_BINS];
+};
+
 /* Pair-kernel argument block (passed by value). */
 struct SimArgs
 {

@@ -1362,7 +1362,8 @@ __device__ __forceinline__ void fill_stale(const SimArgs& a, const Lds& L, int q
       wave_sync(L.lgbuf != nullptr);
       return;
    }
-   const int lo = a.seg_lo ? a.seg_lo[q] : 0;
+   /* clamped into [0, q] like the link kernel's, so no read leaves the block */
+   const int lo = a.seg_lo ? min(max(a.seg_lo[q], 0), q) : 0;
    int v = N;
    int qp = q;
    while ( v < S )

@@ -68,7 +68,10 @@ __global__ __launch_bounds__(256) void k_prev_greater(const int32_t* N, const in
 {
    const int q = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
    if ( q >= count ) return;
-   const int lo = seg_lo ? seg_lo[q] : 0;
+   /* a segment start outside [0, q] is clamped into it, so that no read
+    * leaves the block's arrays (the host entry point rejects such input;
+    * the device entry point passes it through) */
+   const int lo = seg_lo ? min(max(seg_lo[q], 0), q) : 0;
    const int d = prev_greater(N, cmax, q, N[q], lo);
    if ( lane_id() == 0 ) prev[q] = d;
 }
@@ -91,19 +94,19 @@ __global__ __launch_bounds__(256) void k_chunk_max(const int32_t* N, int count, 
 /*
  * Bin the pairs of a block by register slots P(N) for the persistent form.
  * Bin b holds pairs whose N needs the kernel instance with slot count
- * binP[b]; pairs below min_samples (no clustering to do) go to the
+ * bins.P[b]; pairs below min_samples (no clustering to do) go to the
  * smallest bin that still fits their N.  counters[nb] must be zero.
  */
-__global__ __launch_bounds__(256) void k_bin_pairs(const int32_t* N, int count, const int32_t* binP,
+__global__ __launch_bounds__(256) void k_bin_pairs(const int32_t* N, int count, BinBounds bins,
                                                    int nb, int32_t* counters, int32_t* lists)
 {
    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
    if ( q >= count ) return;
    const int n = N[q];
    int b = nb - 1;
-   for ( int i = 0; i < nb; ++i )
+   for ( int i = 0; i < nb && i < KINC_MAX_BINS; ++i )
    {
-      if ( n <= 64 * binP[i] )
+      if ( n <= 64 * bins.P[i] )
       {
          b = i;
          break;

@@ -47,6 +47,8 @@ def lib():
             "oracle_corr_fp64_block": (ctypes.c_int, [P, P, i32, i32, i64, i64, P, P, P]),
             "oracle_math_array": (None, [ctypes.c_int, P, P, i64]),
             "oracle_execute_pairs": (ctypes.c_int, [P, P, i32, i32, P, P, i64, P, P, P, P]),
+            "oracle_execute_pairs_range": (ctypes.c_int, [P, P, i32, i32, P, P, i64, i64, i64,
+                                                          P, P, P, P]),
             "oracle_rmt_maximums": (None, [P, ctypes.c_int, ctypes.c_int, P]),
             "oracle_rmt_prune": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, f32, P, P]),
             "oracle_rmt_eigenvalues": (None, [P, ctypes.c_int, P]),
@@ -274,3 +276,53 @@ def rmt_run(matrix, start=0.99, step=0.001, stop=0.5, chi1=99.607, chi2=200.0, m
                               lc.ctypes.data, cap, ctypes.byref(ft), ctypes.byref(steps))
     n = min(steps.value, cap)
     return rc, ft.value, list(zip(lt[:n].tolist(), ls[:n].tolist(), lc[:n].tolist()))
+
+
+def block_pairs(start, count):
+    """(count, 2) int32 pair list (x, y) of the linear pairs [start, start + count)
+    in index order (Pairwise::Index, src/core/pairwise_index.cpp:30-94)."""
+    k = np.arange(start, start + count, dtype=np.int64)
+    x = ((1 + np.sqrt(1 + 8 * k.astype(np.float64))) // 2).astype(np.int64)
+    x -= (x * (x - 1) // 2 > k)
+    x += ((x + 1) * x // 2 <= k)
+    y = k - x * (x - 1) // 2
+    return np.ascontiguousarray(np.stack([x, y], 1).astype(np.int32))
+
+
+def execute_pairs_threads(params, emx, pairs, seg_lo=None, threads=8, want_labels=True):
+    """execute_pairs split into contiguous ranges on host threads (each
+    range replays the fetches of the earlier pairs of its work block, so the
+    results equal one serial call's)."""
+    from concurrent.futures import ThreadPoolExecutor
+    emx = np.ascontiguousarray(emx, np.float32)
+    G, S = emx.shape
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    count = pairs.shape[0]
+    MC = params.max_clusters
+    K = np.empty(count, np.int8)
+    labels = np.empty((count, S), np.int8) if want_labels else None
+    corr = np.empty((count, MC), np.float32)
+    iters = np.empty((count, MC), np.uint16)
+    seg = None if seg_lo is None else np.ascontiguousarray(seg_lo, np.int32)
+    # small ranges, handed out in order, balance pairs of uneven cost
+    step = max(1, min(256, count // max(1, 4 * threads)))
+    ranges = [(a, min(step, count - a)) for a in range(0, count, step)]
+
+    def run(r):
+        a, n = r
+        rc = lib().oracle_execute_pairs_range(
+            ctypes.byref(params), emx.ctypes.data, G, S, pairs.ctypes.data, _ptr(seg), count,
+            a, n, K[a:].ctypes.data, labels[a:].ctypes.data if labels is not None else None,
+            corr[a:].ctypes.data, iters[a:].ctypes.data)
+        if rc != 0:
+            raise RuntimeError("oracle_execute_pairs_range failed: %d" % rc)
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(run, ranges))
+    return K, labels, corr, iters
+
+
+def execute_block_threads(params, emx, start, count, threads=8, want_labels=True):
+    """execute_block on host threads (see execute_pairs_threads)."""
+    return execute_pairs_threads(params, emx, block_pairs(start, count), None, threads,
+                                 want_labels)

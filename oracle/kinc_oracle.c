@@ -846,10 +846,33 @@ int oracle_execute_block(const kinc_sim_params* p, const float* emx,
  * all 0) is the first pair of q's work block: every block starts with a
  * fresh, zeroed X workspace, exactly as separate Serial::execute calls do.
  */
+int oracle_execute_pairs_range(const kinc_sim_params* p, const float* emx, int32_t G, int32_t S,
+                               const int32_t* pairs, const int32_t* seg_lo, int64_t count,
+                               int64_t first, int64_t n, int8_t* K_out, int8_t* labels_out,
+                               float* corr_out, uint16_t* iters_out);
+
 int oracle_execute_pairs(const kinc_sim_params* p, const float* emx, int32_t G, int32_t S,
                          const int32_t* pairs, const int32_t* seg_lo, int64_t count,
                          int8_t* K_out, int8_t* labels_out, float* corr_out, uint16_t* iters_out)
 {
+   return oracle_execute_pairs_range(p, emx, G, S, pairs, seg_lo, count, 0, count,
+                                     K_out, labels_out, corr_out, iters_out);
+}
+
+/*
+ * Pairs [first, first + n) of the call above, with outputs for those pairs
+ * only.  The X workspace a pair reads depends on the fetches of the earlier
+ * pairs of its work block alone (fetchPair writes it; clustering and
+ * correlation only read it), so the pairs before `first` are fetched and not
+ * clustered.  Disjoint ranges of one call can so run on separate threads and
+ * give the results of the whole call.
+ */
+int oracle_execute_pairs_range(const kinc_sim_params* p, const float* emx, int32_t G, int32_t S,
+                               const int32_t* pairs, const int32_t* seg_lo, int64_t count,
+                               int64_t first, int64_t n, int8_t* K_out, int8_t* labels_out,
+                               float* corr_out, uint16_t* iters_out)
+{
+   if ( first < 0 || n < 0 || first + n > count ) return KINC_E_INVALID;
    const int MC = p->max_clusters;
    const int NP2 = next_power2(S);
    vec2* X = (vec2*)calloc((size_t)S, sizeof(vec2));
@@ -860,21 +883,27 @@ int oracle_execute_pairs(const kinc_sim_params* p, const float* emx, int32_t G, 
    float* wy = (float*)malloc(sizeof(float) * (size_t)NP2);
    float* wr = (float*)malloc(sizeof(float) * (size_t)NP2);
    (void)G;
-   for ( int64_t q = 0; q < count; ++q )
+   /* the latest work-block start at or before `first` */
+   int64_t q0 = first;
+   if ( seg_lo && first < count ) q0 = seg_lo[first];
+   else if ( !seg_lo ) q0 = 0;
+   for ( int64_t q = q0; q < first + n; ++q )
    {
       const int32_t lo = seg_lo ? seg_lo[q] : 0;
-      if ( q == lo ) memset(X, 0, sizeof(vec2) * (size_t)S);   /* a new work block */
+      if ( q == lo || q == q0 ) memset(X, 0, sizeof(vec2) * (size_t)S);   /* a new work block */
       const float* g1 = emx + (size_t)pairs[2 * q] * (size_t)S;
       const float* g2 = emx + (size_t)pairs[2 * q + 1] * (size_t)S;
       int N = fetch_pair(g1, g2, S, p->min_expr, X, labels);
-      uint16_t* it = iters_out ? iters_out + q * MC : NULL;
+      if ( q < first ) continue;   /* workspace replay only */
+      const int64_t o = q - first;
+      uint16_t* it = iters_out ? iters_out + o * MC : NULL;
       if ( it ) memset(it, 0, sizeof(uint16_t) * (size_t)MC);
       int K = 1;
       if ( p->clus_method != KINC_CLUS_NONE )
       {
          K = clustering_compute(p, X, N, labels, wlabels, work, it);
       }
-      float* corr = corr_out + q * MC;
+      float* corr = corr_out + o * MC;
       for ( int k = 0; k < MC; ++k ) corr[k] = NAN;
       for ( int k = 0; k < K; ++k )
       {
@@ -883,8 +912,8 @@ int oracle_execute_pairs(const kinc_sim_params* p, const float* emx, int32_t G, 
                                : oracle_pearson(X, labels, S, (int8_t)k, p->min_samples))
             : oracle_spearman(X, labels, S, (int8_t)k, p->min_samples, wx, wy, wr);
       }
-      K_out[q] = (int8_t)K;
-      if ( labels_out ) memcpy(labels_out + q * S, labels, (size_t)S);
+      K_out[o] = (int8_t)K;
+      if ( labels_out ) memcpy(labels_out + o * S, labels, (size_t)S);
    }
    free(X); free(labels); free(wlabels); free(work);
    free(wx); free(wy); free(wr);

@@ -417,3 +417,25 @@ def test_block_stale_x_pearson():
     # the aligned (non-stale) reading gives a clearly different value
     aligned = np.corrcoef(emx2[2, 6:].astype(np.float64), emx2[0, 6:].astype(np.float64))[0, 1]
     assert abs(aligned - want) > 1e-2
+
+
+# ------------------------------------------------- ranges on host threads
+
+@pytest.mark.parametrize("clus", ["gmm", "kmeans", "none"])
+def test_threaded_ranges_equal_one_serial_call(clus):
+    # Pearson reads the block's shared X workspace, so this also checks that
+    # a range replays the earlier pairs' fetches of its own work block only
+    from kinc_amd import gemgen
+    from kinc_amd import _native as native
+    from parity_util import assert_block_equal
+    x = gemgen.modules(60, 90, seed=5, nan_rate=0.3)
+    p = native.make_params(clusmethod=clus, corrmethod="pearson", crit="BIC", maxclus=3)
+    want = oracle.execute_block(p, x, 40, 700)
+    got = oracle.execute_block_threads(p, x, 40, 700, threads=3)
+    assert_block_equal(got, want, 90, "threaded block")
+    rng = np.random.default_rng(2)
+    pairs = oracle.block_pairs(0, 1770)[rng.permutation(1770)[:600]]
+    seg = np.repeat(np.array([0, 130, 131, 400], np.int32), [130, 1, 269, 200])
+    want = oracle.execute_pairs(p, x, pairs, seg)
+    got = oracle.execute_pairs_threads(p, x, pairs, seg, threads=4)
+    assert_block_equal(got, want, 90, "threaded pair list")
