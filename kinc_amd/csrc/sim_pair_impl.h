@@ -827,6 +827,9 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
             wave_sync(L.lgbuf != nullptr);
             int len = N - m * WAVE;
             len = len < TILE ? len : TILE;
+            /* opaque to loop-invariant hoisting: the partial-tile values of
+             * every unrolled slot would otherwise live across the EM loop */
+            asm volatile("" : "+s"(len));
             acc = chain_add(tile, STR, 1 + 3 * K, len, acc);
             wave_sync(L.lgbuf != nullptr);
          }
@@ -913,6 +916,9 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
             wave_sync(L.lgbuf != nullptr);
             int len = N - m * WAVE;
             len = len < TILE ? len : TILE;
+            /* opaque to loop-invariant hoisting: the partial-tile values of
+             * every unrolled slot would otherwise live across the EM loop */
+            asm volatile("" : "+s"(len));
             accB = chain_add(tile, STR, 3 * K, len, accB);
             wave_sync(L.lgbuf != nullptr);
          }
@@ -991,6 +997,9 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
             wave_sync(L.lgbuf != nullptr);
             int len = N - m * WAVE;
             len = len < TILE ? len : TILE;
+            /* opaque to loop-invariant hoisting: the partial-tile values of
+             * every unrolled slot would otherwise live across the EM loop */
+            asm volatile("" : "+s"(len));
             accE = chain_add(tile, STR, 1, len, accE);
             wave_sync(L.lgbuf != nullptr);
          }
@@ -1724,25 +1733,31 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
 template <int P, int KMAX, int W>
 __global__ __launch_bounds__(64, W) void k_pairs(SimArgs a)
 {
-   if ( a.list == nullptr )
-   {
-      /* workgroups stride over the launch: the grid is one workgroup per
-       * pair, or capped (the memory-store instance's scratch slices, the
-       * persistent form without clustering) */
-      for ( int q = a.first + (int)blockIdx.x; q < a.first + a.launch_count; q += (int)gridDim.x )
-      {
-         process_pair<P, KMAX>(a, q);
-      }
-      return;
-   }
-   const int len = *a.list_len;
+   /* one loop for both: the next pair is the workgroup's next stride (no
+    * list) or the next id popped from the bin's queue, so the per-pair code
+    * is inlined once */
+   const bool queued = a.list != nullptr;
+   const int len = queued ? *a.list_len : 0;
+   const int end = a.first + a.launch_count;
+   int q = a.first + (int)blockIdx.x;
    for ( ;; )
    {
-      int idx = 0;
-      if ( lane_id() == 0 ) idx = atomicAdd(a.list_head, 1);
-      idx = bcast_i(idx, 0);
-      if ( idx >= len ) break;
-      process_pair<P, KMAX>(a, a.list[idx]);
+      int pair;
+      if ( !queued )
+      {
+         if ( q >= end ) break;
+         pair = q;
+         q += (int)gridDim.x;
+      }
+      else
+      {
+         int idx = 0;
+         if ( lane_id() == 0 ) idx = atomicAdd(a.list_head, 1);
+         idx = bcast_i(idx, 0);
+         if ( idx >= len ) break;
+         pair = a.list[idx];
+      }
+      process_pair<P, KMAX>(a, pair);
    }
 }
 
