@@ -145,15 +145,25 @@ def slots_for(n, S=500):
     return 0
 
 
-def run_form(native, x, flags, form, blocks, warmup, torch, workers=2):
+def run_form(native, x, flags, form, blocks, warmup, torch, workers=2, autotune=False):
     """Time the blocks on `workers` engine handles taking blocks in turn (the
     reference's several workers per device): each handle has its own stream,
-    so a block's bins start while the previous block's last pairs finish."""
+    so a block's bins start while the previous block's last pairs finish.
+    With autotune, the first handle times its launch configurations on the
+    first 8192 pairs of the first block (kinc_sim_autotune) and every handle
+    takes the fastest; untimed, before the warm-up."""
     import ctypes  # noqa: F401
     G, S = x.shape
     total_pairs = native.pair_count(G)
     p = native.make_params(form=form, **flags)
     engs = [native.Engine(x, p) for _ in range(workers)]
+    tuning = engs[0].tuning()
+    if autotune:
+        start = blocks[0] * BLOCK
+        tuning = engs[0].autotune(start, min(8192, total_pairs - start))
+        for e in engs[1:]:
+            e.set_tuning(tuning["w5"], tuning["aux_streams"])
+        log("  %s autotune: %s" % (form, tuning))
     MC = p.max_clusters
     nsteps = len(blocks)
     dev = torch.device("cuda:0")
@@ -197,6 +207,7 @@ def run_form(native, x, flags, form, blocks, warmup, torch, workers=2):
         iters=dI.cpu().numpy().view(np.uint16))
     for e in engs:
         e.close()
+    out["tuning"] = tuning
     return pairs, t1 - t0, prof, out
 
 
@@ -295,10 +306,14 @@ def cpu_baseline_threads(oracle, native, x, flags, blocks, seconds, gpu_out, rat
 
 
 def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
-    """Poisson query stream against the resident 60,000 x 475 matrix (GMM +
-    Pearson, |r| >= 0.5): query sizes log-uniform on [1, max_genes] genes,
-    all partners of each query gene; latency is submit-to-answer as measured
-    by the server.  Returns the `serving` object of the JSON line."""
+    """Config D's serving load: a Poisson query stream against the resident
+    60,000 x 475 matrix (GMM + Pearson, |r| >= 0.5), query sizes log-uniform
+    on [1, max_genes] genes, all partners of each query gene; latency is
+    submit-to-answer as measured by the server, and start wait the part of it
+    before the query's first work block was launched.  Then a 1-gene query
+    submitted while a big_query-gene query runs shows how long a small query
+    waits behind a large one (at most the two one-block batches in flight).
+    Returns the `serving` object of the JSON line."""
     import threading
     from kinc_amd import serving
     cfg = gemgen.CONFIGS["D"]
@@ -307,7 +322,8 @@ def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
     G = x.shape[0]
     log("serving: generated %dx%d GEM in %.1fs" % (G, x.shape[1], time.time() - t))
     p = native.make_params(**cfg["params"])
-    srv = serving.Server(x, p, hbm_budget=8 << 30, batch_pairs=131072)
+    quantum = 32768
+    srv = serving.Server(x, p, hbm_budget=16 << 30, batch_pairs=quantum)
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(0.0, np.log(max_genes), n_queries)).astype(int).clip(1, max_genes)
     queries = [rng.choice(G, size=int(k), replace=False) for k in sizes]
@@ -326,7 +342,7 @@ def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
     t0 = time.perf_counter()
     th = threading.Thread(target=producer)
     th.start()
-    lat, npairs, nhits = [], 0, 0
+    lat, waits, npairs, nhits, per_query = [], [], 0, 0, []
     done = 0
     while done < n_queries:
         with lock:
@@ -336,13 +352,17 @@ def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
             continue
         qid = pending[0]
         assert srv.wait(qid, 1200)
+        w, r = srv.timing(qid)
         hits, l, n = srv.result(qid)
         lat.append(l)
+        waits.append(w)
         npairs += n
         nhits += len(hits)
+        per_query.append({"genes": int(sizes[done]), "pairs": int(n), "latency_s": l,
+                          "start_wait_s": w})
         done += 1
-        log("  query %d/%d: %d genes, %d pairs, %d hits, %.3f s" % (done, n_queries, sizes[done - 1],
-                                                                   n, len(hits), l))
+        log("  query %d/%d: %d genes, %d pairs, %d hits, %.3f s (start wait %.3f s)"
+            % (done, n_queries, sizes[done - 1], n, len(hits), l, w))
     th.join()
     wall = time.perf_counter() - t0
     p1, b1, busy1 = srv.stats()
@@ -350,18 +370,31 @@ def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
         "workload": "60000 genes x 475 samples FP32 (module generator, 8% NaN), GMM K=1..5 BIC + "
                     "Pearson, |r| >= 0.5, all partners of each query gene",
         "queries": n_queries, "qps": qps, "query_genes": "log-uniform on [1, %d]" % max_genes,
+        "quantum_pairs": quantum,
         "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
         "max_s": float(np.max(lat)),
+        "start_wait_p50_s": float(np.percentile(waits, 50)),
+        "start_wait_p99_s": float(np.percentile(waits, 99)),
         "pairs_per_s": (p1 - p0) / wall, "device_busy_frac": (busy1 - busy0) / wall,
         "pairs": int(npairs), "hits": int(nhits), "batches": int(b1 - b0),
+        "per_query": per_query,
     }
     if big_query:
-        qgenes = rng.choice(G, size=big_query, replace=False)
-        qid = srv.submit(qgenes, 0.5)
-        assert srv.wait(qid, 1200)
-        hits, l, n = srv.result(qid)
-        log("  %d-gene query alone: %d pairs, %.2f s" % (big_query, n, l))
-        out["single_query"] = {"genes": big_query, "pairs": int(n), "latency_s": l}
+        qbig = srv.submit(rng.choice(G, size=big_query, replace=False), 0.5)
+        time.sleep(1.0)
+        qone = srv.submit(rng.choice(G, size=1, replace=False), 0.5)
+        assert srv.wait(qone, 1200)
+        w1, r1 = srv.timing(qone)
+        _, l1, n1 = srv.result(qone)
+        assert srv.wait(qbig, 1200)
+        wb, rb = srv.timing(qbig)
+        hb, lb, nb = srv.result(qbig)
+        log("  %d-gene query: %d pairs, %.2f s; 1-gene query submitted 1 s into it: %d pairs, "
+            "%.3f s (start wait %.3f s)" % (big_query, nb, lb, n1, l1, w1))
+        out["single_query"] = {"genes": big_query, "pairs": int(nb), "latency_s": lb}
+        out["small_behind_large"] = {"genes": 1, "pairs": int(n1), "latency_s": l1,
+                                     "start_wait_s": w1, "run_s": r1,
+                                     "behind": "%d-gene query submitted 1 s earlier" % big_query}
     srv.close()
     return out
 
@@ -463,10 +496,12 @@ def main():
     ap.add_argument("--pearson-fp64", action="store_true",
                     help="Pearson sums and quotient in double (the pearson_fp64 option)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the handles' default launch configuration (no kinc_sim_autotune)")
     ap.add_argument("--no-rmt", action="store_true", help="skip the RMT threshold-sweep leg")
-    ap.add_argument("--serve-queries", type=int, default=24)
-    ap.add_argument("--serve-qps", type=float, default=0.5)
-    ap.add_argument("--serve-max-genes", type=int, default=16)
+    ap.add_argument("--serve-queries", type=int, default=8)
+    ap.add_argument("--serve-qps", type=float, default=0.1)
+    ap.add_argument("--serve-max-genes", type=int, default=500)
     ap.add_argument("--serve-big-query", type=int, default=100)
     ap.add_argument("--valu-calibration", action="store_true",
                     help="only run the VALU issue-rate probe (v_add_f32 streams, 4 waves/SIMD)")
@@ -533,7 +568,7 @@ def main():
         blocks = list(range(min(args.steps, nblocks)))
 
     pairs, secs, prof, out = run_form(native, x, flags, "persistent", blocks, args.warmup, torch,
-                                      args.workers)
+                                      args.workers, autotune=not args.no_autotune)
     value = pairs / secs
     log("persistent: %d pairs in %.3fs -> %.0f pairs/s; kernels %s" % (pairs, secs, value, prof))
 
@@ -671,6 +706,7 @@ def main():
             "parallelism": "1 GPU",
             "workers": args.workers,
             "pearson_fp64": bool(args.pearson_fp64),
+            "tuning": out.get("tuning"),
         },
         "plain_value": plain_value,
         "batched_vs_plain": (value / plain_value) if plain_value else None,

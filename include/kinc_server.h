@@ -13,12 +13,16 @@
  * 32,768 pairs, the unit of the reference's stale-buffer Pearson semantics.
  *
  * Continuous batching: a scheduler thread admits queued queries in arrival
- * order while their device result buffers fit in the HBM budget, then packs
- * whole work blocks of all admitted queries, round-robin, into batches of up
- * to batch_pairs pairs, each one persistent-kernel launch; a query that
- * arrives while others are running joins the next batch.  Two batches are
- * in flight; results are the same whether a query runs alone or mixed with
- * others.
+ * order, each one whose device result buffer fits in the HBM budget (a query
+ * that does not fit holds back the later ones only after it has waited 5 s),
+ * then packs whole work blocks of the admitted queries into batches, each
+ * one persistent-kernel launch: fewest remaining blocks first, one work
+ * block per batch while more than one query has work or waits (the
+ * scheduling quantum), up to batch_pairs pairs while one query runs alone.
+ * A query that arrives while others run joins the next batch, so it waits
+ * at most for the two batches in flight (one on each of the server's two
+ * engine handles).  Results are the same whether a query runs alone or
+ * mixed with others.
  */
 #ifndef KINC_SERVER_H
 #define KINC_SERVER_H
@@ -66,6 +70,10 @@ int kinc_server_wait(kinc_server* s, int64_t id, double timeout_s);
  */
 int kinc_server_result(kinc_server* s, int64_t id, kinc_hit* hits, int64_t cap,
                        int64_t* n_hits, double* latency_s, int64_t* n_pairs);
+
+/* Timing of a finished query (before kinc_server_result releases it):
+ * submit to the launch of its first block, and from there to completion. */
+int kinc_server_timing(kinc_server* s, int64_t id, double* start_wait_s, double* run_s);
 
 /* Totals since start: pairs evaluated, batches launched, seconds with a
  * batch on the device. */

@@ -179,6 +179,32 @@ int kinc_sim_set_profiling(kinc_sim* h, int32_t enabled);
 int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launches,
                           double* total_ms, int32_t cap);
 
+/*
+ * Launch configuration of a handle (instrumentation and tuning; the
+ * reference's only performance knob is --ksize, src/core/similarity_input.cpp:
+ * 241-251).  Every configuration gives the same results.
+ *   w5           1: the bins of at most 6 register slots run their builds for
+ *                5 waves per SIMD (only when the per-wave LDS leaves room for
+ *                20 waves per CU); 0: the 4-wave builds
+ *   aux_streams  streams the persistent form's bins run on concurrently (1-4)
+ * kinc_sim_autotune runs the sample of pairs [start, start+count) (at most
+ * one work block) with every candidate configuration, three times each,
+ * checks that all of them give identical K and correlations, keeps the
+ * fastest on the handle and reports it (candidates tried, best time of the
+ * sample in ms).  Results of later calls are unchanged by it.
+ */
+typedef struct kinc_sim_tuning
+{
+   int32_t w5;
+   int32_t aux_streams;
+   int32_t candidates;
+   double sample_ms;
+} kinc_sim_tuning;
+
+int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc_sim_tuning* out);
+int kinc_sim_get_tuning(const kinc_sim* h, kinc_sim_tuning* out);
+int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t);
+
 /* Number of linear pairs for G genes: G(G-1)/2 (src/core/similarity.cpp:19). */
 int64_t kinc_pair_count(int32_t gene_size);
 

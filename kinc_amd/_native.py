@@ -71,6 +71,9 @@ SIGNATURES = [
     ("kinc_selftest_math", ctypes.c_int, [ctypes.c_int32, _P, _P, ctypes.c_int64]),
     ("kinc_selftest_valu_probe", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+    ("kinc_sim_autotune", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P]),
+    ("kinc_sim_get_tuning", ctypes.c_int, [_P, _P]),
+    ("kinc_sim_set_tuning", ctypes.c_int, [_P, _P]),
     ("kinc_pair_count", ctypes.c_int64, [ctypes.c_int32]),
     ("kinc_pair_index", None, [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
                                ctypes.POINTER(ctypes.c_int32)]),
@@ -120,6 +123,16 @@ def make_params(clusmethod="none", corrmethod="pearson", minexpr=float("-inf"), 
     p.batch_capacity = batch_capacity
     p.pearson_fp64 = int(bool(pearson_fp64))
     return p
+
+
+class Tuning(ctypes.Structure):
+    """Mirror of kinc_sim_tuning (include/kinc_sim.h)."""
+    _fields_ = [("w5", ctypes.c_int32), ("aux_streams", ctypes.c_int32),
+                ("candidates", ctypes.c_int32), ("sample_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"w5": bool(self.w5), "aux_streams": int(self.aux_streams),
+                "candidates": int(self.candidates), "sample_ms": float(self.sample_ms)}
 
 
 class KincError(RuntimeError):
@@ -214,6 +227,24 @@ class Engine:
 
     def synchronize(self):
         self._check(lib().kinc_sim_synchronize(self.h))
+
+    def autotune(self, start, count):
+        """Time the handle's launch configurations on pairs [start, start+count)
+        and keep the fastest (kinc_sim_autotune); returns it as a dict."""
+        t = Tuning()
+        self._check(lib().kinc_sim_autotune(self.h, start, count, ctypes.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().kinc_sim_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, w5, aux_streams):
+        t = Tuning()
+        t.w5 = 1 if w5 else 0
+        t.aux_streams = aux_streams
+        self._check(lib().kinc_sim_set_tuning(self.h, ctypes.byref(t)))
 
     def close(self):
         if getattr(self, "h", None):

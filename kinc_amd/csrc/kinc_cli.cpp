@@ -11,7 +11,9 @@
  *        [--mincorr F] [--maxcorr F] [--ksize N]
  * plus flags the reference does not have: --device N, --form
  * plain|persistent, --pearson-fp64 BOOL (Pearson sums and quotient in
- * double) and --workers 1|2 (engine handles taking blocks in turn); and the
+ * double), --workers 1|2 (engine handles taking blocks in turn) and
+ * --autotune BOOL (default true: time the launch configurations on a sample
+ * first, kinc_sim_autotune; results are the same either way); and the
  * expression-matrix converters around it (importexpressionmatrix_input.cpp,
  * exportexpressionmatrix_input.cpp):
  *   kinc run import-emx --input A.txt --output A.emx [--nan TOKEN]
@@ -35,6 +37,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -59,7 +62,7 @@ int usage()
            "       [--corrmethod pearson|spearman] [--minexpr F] [--minsamp N] [--minclus N]\n"
            "       [--maxclus N] [--crit BIC|ICL] [--preout BOOL] [--postout BOOL]\n"
            "       [--mincorr F] [--maxcorr F] [--ksize N] [--device N] [--form plain|persistent]\n"
-           "       [--pearson-fp64 BOOL] [--workers 1|2]\n"
+           "       [--pearson-fp64 BOOL] [--workers 1|2] [--autotune BOOL]\n"
            "       kinc run import-emx --input TXT --output EMX [--nan TOKEN] [--samples N]\n"
            "       [--transform none|'natural logarithm'|'logarithm base 2'|'logarithm base 10']\n"
            "       [--threads N]\n"
@@ -229,6 +232,7 @@ int main(int argc, char** argv)
    std::string in, ccm, cmx;
    int device = 0;
    int workers = 2;
+   int autotune = 1;
    for ( int i = 3; i < argc; i += 2 )
    {
       if ( i + 1 >= argc ) return usage();
@@ -250,6 +254,7 @@ int main(int argc, char** argv)
       else if ( k == "--ksize" ) p.ksize = atoi(v.c_str());
       else if ( k == "--device" ) device = atoi(v.c_str());
       else if ( k == "--workers" ) workers = atoi(v.c_str());
+      else if ( k == "--autotune" ) autotune = parse_bool(v);
       else if ( k == "--form" ) p.form = v == "plain" ? KINC_FORM_PLAIN : KINC_FORM_PERSISTENT;
       else if ( k == "--pearson-fp64" ) p.pearson_fp64 = parse_bool(v);
       else return usage();
@@ -293,6 +298,20 @@ int main(int argc, char** argv)
          for ( int j = 0; j < k; ++j ) kinc_sim_destroy(hs[j]);
          return 1;
       }
+   }
+   /* launch configuration: timed on the first block's first 8192 pairs by
+    * the first handle, then used by all (results do not depend on it) */
+   if ( autotune )
+   {
+      const int64_t n = std::min<int64_t>(8192, kinc_pair_count(G));
+      kinc_sim_tuning t;
+      if ( kinc_sim_autotune(hs[0], 0, n, &t) != KINC_OK )
+      {
+         fprintf(stderr, "%s\n", kinc_sim_last_error(hs[0]));
+         for ( int k = 0; k < workers; ++k ) kinc_sim_destroy(hs[k]);
+         return 1;
+      }
+      for ( int k = 1; k < workers; ++k ) kinc_sim_set_tuning(hs[k], &t);
    }
    kinc_similarity_writer* w = nullptr;
    if ( kinc_similarity_writer_open(ccm.c_str(), cmx.c_str(), G, S, p.max_clusters, p.corr_method,

@@ -142,6 +142,7 @@ struct kinc_sim
    /* the persistent form drains its bins concurrently on these streams */
    static const int kAux = 4;
    hipStream_t aux[kAux] = {};
+   int naux = kAux;                /* auxiliary streams the bins are spread over (1..kAux) */
    hipEvent_t binned = nullptr;
    hipEvent_t drained[kAux] = {};
    SimArgs base{};
@@ -152,6 +153,7 @@ struct kinc_sim
    /* the per-wave LDS leaves room for 20 waves per CU: small instances run
     * their 5-waves-per-SIMD builds */
    bool w5 = false;
+   bool w5_fits = false;           /* the per-wave LDS allows 20 waves per CU */
    struct Timed { int P; hipEvent_t a, b; };
    std::vector<Timed> timed;
    std::vector<hipEvent_t> event_pool;
@@ -325,7 +327,8 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       a.lds_tile = a.lds_u;
       off += std::max(tile_bytes, sort_bytes);
       h->lds_bytes = align16(off);
-      h->w5 = h->lds_bytes * 20 <= (size_t)160 * 1024;
+      h->w5_fits = h->lds_bytes * 20 <= (size_t)160 * 1024;
+      h->w5 = h->w5_fits;
    }
    else
    {
@@ -653,15 +656,16 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
     * auxiliary streams (the allocation-free entry point) they run one after
     * another on s. */
    const bool fork = h->aux[0] != nullptr;
+   const int naux = fork ? std::max(1, std::min(h->naux, (int)kinc_sim::kAux)) : 1;
    if ( fork )
    {
       HIP_TRY(h, hipEventRecord(h->binned, s));
-      for ( int i = 0; i < kinc_sim::kAux; ++i ) HIP_TRY(h, hipStreamWaitEvent(h->aux[i], h->binned, 0));
+      for ( int i = 0; i < naux; ++i ) HIP_TRY(h, hipStreamWaitEvent(h->aux[i], h->binned, 0));
    }
    for ( int b = h->nbins - 1; b >= 0; --b )
    {
       const int P = h->classes[b];
-      hipStream_t sb = fork ? h->aux[(h->nbins - 1 - b) % kinc_sim::kAux] : s;
+      hipStream_t sb = fork ? h->aux[(h->nbins - 1 - b) % naux] : s;
       const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
       if ( !fn ) return fail_c(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
@@ -677,7 +681,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       int rc = launch_pairs(h, P, fn, (unsigned)grid, args, sb);
       if ( rc ) return rc;
    }
-   for ( int i = 0; fork && i < kinc_sim::kAux; ++i )
+   for ( int i = 0; fork && i < naux; ++i )
    {
       HIP_TRY(h, hipEventRecord(h->drained[i], h->aux[i]));
       HIP_TRY(h, hipStreamWaitEvent(s, h->drained[i], 0));
@@ -816,4 +820,130 @@ extern "C" void kinc_sim_destroy(kinc_sim* h)
 extern "C" const char* kinc_sim_last_error(const kinc_sim* h)
 {
    return h ? h->err : g_create_error;
+}
+
+/* ------------------------------------------------------------- autotuner */
+
+/*
+ * The launch configurations a handle can run with and that give the same
+ * results: the 5-waves-per-SIMD builds of the small register-slot instances
+ * (on or off) and how many auxiliary streams the persistent form's bins are
+ * spread over.  kinc_sim_autotune times each on a sample of pairs and keeps
+ * the fastest.
+ */
+static int time_config(kinc_sim* h, int64_t start, int count, bool w5, int naux, double* ms)
+{
+   h->w5 = w5;
+   h->naux = naux;
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   HIP_TRY(h, hipEventCreate(&e0));
+   int rc = KINC_OK;
+   if ( hipEventCreate(&e1) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventCreate failed");
+   double best = 1e30;
+   for ( int rep = 0; rep < 3 && rc == KINC_OK; ++rep )
+   {
+      /* the first run warms up (code objects, caches); the best of the others counts */
+      if ( hipEventRecord(e0, h->stream) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventRecord failed");
+      if ( rc == KINC_OK )
+      {
+         rc = run(h, start, nullptr, nullptr, count, h->d_K, nullptr, h->d_corr, nullptr, h->stream);
+      }
+      if ( rc == KINC_OK && hipEventRecord(e1, h->stream) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventRecord failed");
+      if ( rc == KINC_OK && hipEventSynchronize(e1) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventSynchronize failed");
+      float t = 0.0f;
+      if ( rc == KINC_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventElapsedTime failed");
+      if ( rc == KINC_OK && rep > 0 ) best = std::min(best, (double)t);
+   }
+   if ( e0 ) (void)hipEventDestroy(e0);
+   if ( e1 ) (void)hipEventDestroy(e1);
+   *ms = best;
+   return rc;
+}
+
+extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc_sim_tuning* out)
+{
+   if ( !h ) return KINC_E_INVALID;
+   if ( count < 1 || count > KINC_WORK_BLOCK_SIZE || count > h->cap )
+   {
+      return fail_c(h, KINC_E_INVALID, "autotune sample must hold 1..min(32768, batch_capacity) pairs");
+   }
+   if ( start < 0 || start + count > kinc_pair_count(h->G) ) return fail_c(h, KINC_E_INVALID, "pair range outside the matrix");
+   int rc = ensure_staging(h);
+   if ( rc ) return rc;
+   HIP_TRY(h, hipSetDevice(h->device));
+   const bool saved_w5 = h->w5;
+   const int saved_naux = h->naux;
+   /* candidates; the 5-wave builds only where the LDS leaves room for them */
+   const bool w5s[2] = {false, true};
+   const int nauxs[3] = {1, 2, 4};
+   bool best_w5 = saved_w5;
+   int best_naux = saved_naux;
+   double best_ms = 1e30;
+   int tried = 0;
+   std::vector<int8_t> K0((size_t)count), K1((size_t)count);
+   std::vector<float> C0((size_t)count * (size_t)h->p.max_clusters), C1(C0.size());
+   for ( bool w5 : w5s )
+   {
+      if ( w5 && !h->w5_fits ) continue;
+      for ( int naux : nauxs )
+      {
+         double ms = 0.0;
+         rc = time_config(h, start, (int)count, w5, naux, &ms);
+         if ( rc ) break;
+         /* every configuration must give the same bits */
+         std::vector<int8_t>& K = tried == 0 ? K0 : K1;
+         std::vector<float>& C = tried == 0 ? C0 : C1;
+         if ( hipMemcpy(K.data(), h->d_K, (size_t)count, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(C.data(), h->d_corr, sizeof(float) * C.size(), hipMemcpyDeviceToHost) != hipSuccess )
+         {
+            rc = fail_c(h, KINC_E_DEVICE, "autotune: copying the sample's results failed");
+            break;
+         }
+         if ( tried > 0 && (std::memcmp(K0.data(), K1.data(), K0.size()) != 0 ||
+                            std::memcmp(C0.data(), C1.data(), sizeof(float) * C0.size()) != 0) )
+         {
+            rc = fail_c(h, KINC_E_DEVICE, "autotune: launch configurations gave different results");
+            break;
+         }
+         ++tried;
+         if ( ms < best_ms )
+         {
+            best_ms = ms;
+            best_w5 = w5;
+            best_naux = naux;
+         }
+      }
+      if ( rc ) break;
+   }
+   h->w5 = rc ? saved_w5 : best_w5;
+   h->naux = rc ? saved_naux : best_naux;
+   if ( rc ) return rc;
+   if ( out )
+   {
+      out->w5 = h->w5 ? 1 : 0;
+      out->aux_streams = h->naux;
+      out->candidates = tried;
+      out->sample_ms = best_ms;
+   }
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_get_tuning(const kinc_sim* h, kinc_sim_tuning* out)
+{
+   if ( !h || !out ) return KINC_E_INVALID;
+   out->w5 = h->w5 ? 1 : 0;
+   out->aux_streams = h->naux;
+   out->candidates = 0;
+   out->sample_ms = 0.0;
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t)
+{
+   if ( !h || !t ) return KINC_E_INVALID;
+   if ( t->aux_streams < 1 || t->aux_streams > kinc_sim::kAux ) return fail_c(h, KINC_E_INVALID, "aux_streams must lie in [1, 4]");
+   if ( t->w5 && !h->w5_fits ) return fail_c(h, KINC_E_UNSUPPORTED, "the per-wave LDS leaves no room for 5 waves per SIMD");
+   h->w5 = t->w5 != 0;
+   h->naux = t->aux_streams;
+   return KINC_OK;
 }

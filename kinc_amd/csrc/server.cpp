@@ -1,17 +1,23 @@
 /*
  * server.cpp — continuous-batching query server (include/kinc_server.h).
  *
- * One scheduler thread owns a kinc_sim handle with the expression matrix
- * resident.  Queries queue in arrival order; a query is admitted when its
- * device result buffer (K and correlations of all its pairs) fits in the
- * HBM budget, which is a first-fit arena carved out of one allocation made
- * at start-up (no allocation on the serving path).  Each batch takes whole
- * 32,768-pair work blocks round-robin from the admitted queries, runs as one
- * persistent-form call with per-block segment starts, and its outputs are
- * scattered into the queries' result buffers.  Two batches are in flight, so
- * the host packs batch b+1 while the device runs batch b.  A query whose
- * blocks are all done is copied back on a separate stream and filtered into
- * hits (Similarity::process thresholds, src/core/similarity.cpp:104-123).
+ * One scheduler thread owns two kinc_sim handles with the expression matrix
+ * resident, each with its own stream.  Queries queue in arrival order; a
+ * query is admitted when its device result buffer (K and correlations of
+ * all its pairs) fits in the HBM budget, which is a first-fit arena carved
+ * out of one allocation made at start-up (no allocation on the serving
+ * path); a query that does not fit does not hold back later ones that do,
+ * until it has waited longer than the aging limit.  Each batch takes whole
+ * 32,768-pair work blocks from the admitted queries, fewest remaining blocks
+ * first, runs as one persistent-form call with per-block segment starts,
+ * and its outputs are scattered into the queries' result buffers.  A batch
+ * is one work block (the scheduling quantum) while more than one query has
+ * work or waits, and up to batch_pairs pairs when one query runs alone.  The
+ * two handles take batches in turn, so a batch starts on the CUs the
+ * previous one's last pairs leave idle, and the host packs the next batch
+ * while the device runs.  A query whose blocks are all done is copied back
+ * on a separate stream and filtered into hits (Similarity::process
+ * thresholds, src/core/similarity.cpp:104-123).
  */
 #include <hip/hip_runtime.h>
 
@@ -102,6 +108,8 @@ struct Query
    size_t corr_off = 0;
    int state = 0;               /* 0 queued, 1 admitted, 2 copying, 3 done, -1 failed */
    Clock::time_point t_submit, t_done;
+   Clock::time_point t_first;   /* launch of its first block */
+   bool started = false;
    std::vector<int8_t> hK;
    std::vector<float> hcorr;
    hipEvent_t copied = nullptr;
@@ -118,6 +126,7 @@ struct Segment
 
 struct Batch
 {
+   int handle = 0;              /* engine handle and stream the batch runs on */
    int32_t* h_pairs = nullptr;  /* pinned */
    int32_t* h_seg = nullptr;    /* pinned */
    int32_t* d_pairs = nullptr;
@@ -141,9 +150,11 @@ struct kinc_server
    int S = 0;
    int MC = 0;
    int cap = 0;
-   kinc_sim* sim = nullptr;
-   hipStream_t stream = nullptr;     /* batches */
+   kinc_sim* sim[2] = {};            /* two handles, one per in-flight batch */
+   hipStream_t stream[2] = {};       /* batches of each handle */
    hipStream_t copy = nullptr;       /* result copy-back */
+   hipEvent_t t0 = nullptr;          /* device time origin of the busy record */
+   double busy_until_ms = 0.0;       /* end of the last counted busy interval */
    char* d_arena = nullptr;
    size_t budget = 0;
    std::unique_ptr<Arena> arena;
@@ -163,8 +174,13 @@ struct kinc_server
    std::atomic<int64_t> pairs_done{0};
    std::atomic<int64_t> batches{0};
    double busy_s = 0.0;
-   size_t rr = 0;                     /* round-robin position */
+   int next_handle = 0;               /* the handle the next batch runs on */
 };
+
+/* A query that does not fit in the HBM budget stops holding back later
+ * queries after this long in the queue; from then on nothing is admitted
+ * ahead of it. */
+static const double kAgingSeconds = 5.0;
 
 static int server_fail(kinc_server* s, int code, const std::string& m)
 {
@@ -248,57 +264,68 @@ static void filter_hits(kinc_server* s, Query& q)
 
 static bool launch_batch(kinc_server* s, Batch& b)
 {
-   /* pack whole blocks of admitted queries, round-robin */
+   /* queries with blocks left, fewest remaining blocks first (arrival order
+    * among equals); a batch is one block while more than one query has work
+    * or another waits for admission, so a new small query waits at most for
+    * the batches already in flight */
    std::vector<std::shared_ptr<Query>> cand;
+   bool others_waiting = false;
    {
       std::lock_guard<std::mutex> g(s->mu);
       for ( auto& q : s->active )
       {
          if ( q->state == 1 && q->next_block < q->nblocks ) cand.push_back(q);
       }
+      others_waiting = !s->pending.empty();
    }
    if ( cand.empty() ) return false;
+   std::stable_sort(cand.begin(), cand.end(), [](const std::shared_ptr<Query>& x,
+                                                 const std::shared_ptr<Query>& y) {
+      const int64_t rx = x->nblocks - x->next_block, ry = y->nblocks - y->next_block;
+      return rx != ry ? rx < ry : x->id < y->id;
+   });
+   const int limit = (cand.size() > 1 || others_waiting) ? KINC_WORK_BLOCK_SIZE : s->cap;
    b.segs.clear();
    b.count = 0;
-   bool progress = true;
-   size_t start = s->rr % cand.size();
-   while ( progress )
+   for ( auto& q : cand )
    {
-      progress = false;
-      for ( size_t t = 0; t < cand.size(); ++t )
+      while ( q->next_block < q->nblocks )
       {
-         auto& q = cand[(start + t) % cand.size()];
-         if ( q->next_block >= q->nblocks ) continue;
          const int64_t lo = q->next_block * KINC_WORK_BLOCK_SIZE;
          const int cnt = (int)std::min<int64_t>(KINC_WORK_BLOCK_SIZE, q->n - lo);
-         if ( b.count + cnt > s->cap ) continue;
+         if ( b.count + cnt > limit ) break;
          memcpy(b.h_pairs + 2 * (size_t)b.count, q->pairs.data() + 2 * lo, sizeof(int32_t) * 2 * (size_t)cnt);
          for ( int i = 0; i < cnt; ++i ) b.h_seg[b.count + i] = b.count;
          b.segs.push_back(Segment{q, q->next_block, b.count, cnt});
+         if ( !q->started )
+         {
+            q->started = true;
+            q->t_first = Clock::now();
+         }
          b.count += cnt;
          q->next_block++;
-         progress = true;
       }
+      if ( b.count >= limit ) break;
    }
-   s->rr++;
    if ( b.count == 0 ) return false;
 
-   SRV_HIP(hipEventRecord(b.start, s->stream));
-   SRV_HIP(hipMemcpyAsync(b.d_pairs, b.h_pairs, sizeof(int32_t) * 2 * (size_t)b.count, hipMemcpyHostToDevice, s->stream));
-   SRV_HIP(hipMemcpyAsync(b.d_seg, b.h_seg, sizeof(int32_t) * (size_t)b.count, hipMemcpyHostToDevice, s->stream));
-   int rc = kinc_sim_execute_pairs_device(s->sim, b.d_pairs, b.d_seg, b.count, b.d_K, nullptr, b.d_corr,
-                                          nullptr, s->stream);
-   if ( rc != KINC_OK ) throw std::string(kinc_sim_last_error(s->sim));
+   const hipStream_t st = s->stream[b.handle];
+   SRV_HIP(hipEventRecord(b.start, st));
+   SRV_HIP(hipMemcpyAsync(b.d_pairs, b.h_pairs, sizeof(int32_t) * 2 * (size_t)b.count, hipMemcpyHostToDevice, st));
+   SRV_HIP(hipMemcpyAsync(b.d_seg, b.h_seg, sizeof(int32_t) * (size_t)b.count, hipMemcpyHostToDevice, st));
+   int rc = kinc_sim_execute_pairs_device(s->sim[b.handle], b.d_pairs, b.d_seg, b.count, b.d_K, nullptr,
+                                          b.d_corr, nullptr, st);
+   if ( rc != KINC_OK ) throw std::string(kinc_sim_last_error(s->sim[b.handle]));
    for ( auto& sg : b.segs )
    {
       const int64_t lo = sg.block * KINC_WORK_BLOCK_SIZE;
       char* base = s->d_arena + sg.q->off;
-      SRV_HIP(hipMemcpyAsync(base + lo, b.d_K + sg.offset, (size_t)sg.count, hipMemcpyDeviceToDevice, s->stream));
+      SRV_HIP(hipMemcpyAsync(base + lo, b.d_K + sg.offset, (size_t)sg.count, hipMemcpyDeviceToDevice, st));
       SRV_HIP(hipMemcpyAsync(base + sg.q->corr_off + sizeof(float) * (size_t)lo * s->MC,
                              b.d_corr + (size_t)sg.offset * s->MC, sizeof(float) * (size_t)sg.count * s->MC,
-                             hipMemcpyDeviceToDevice, s->stream));
+                             hipMemcpyDeviceToDevice, st));
    }
-   SRV_HIP(hipEventRecord(b.done, s->stream));
+   SRV_HIP(hipEventRecord(b.done, st));
    b.busy = true;
    s->batches++;
    return true;
@@ -307,11 +334,17 @@ static bool launch_batch(kinc_server* s, Batch& b)
 /* A finished batch: count its blocks, start the copy-back of complete queries. */
 static void retire_batch(kinc_server* s, Batch& b)
 {
-   float ms = 0.0f;
-   if ( hipEventElapsedTime(&ms, b.start, b.done) == hipSuccess )
+   /* device-busy time: the union of the batches' intervals (batches on the
+    * two handles overlap), measured from the server's time origin; batches
+    * retire in start order */
+   float t_start = 0.0f, t_end = 0.0f;
+   if ( hipEventElapsedTime(&t_start, s->t0, b.start) == hipSuccess &&
+        hipEventElapsedTime(&t_end, s->t0, b.done) == hipSuccess )
    {
+      const double from = std::max((double)t_start, s->busy_until_ms);
       std::lock_guard<std::mutex> g(s->mu);   /* kinc_server_stats reads it */
-      s->busy_s += ms * 1e-3;
+      if ( t_end > from ) s->busy_s += (t_end - from) * 1e-3;
+      s->busy_until_ms = std::max(s->busy_until_ms, (double)t_end);
    }
    s->pairs_done += b.count;
    for ( auto& sg : b.segs )
@@ -344,14 +377,15 @@ static void scheduler(kinc_server* s)
       for ( ;; )
       {
          bool did = false;
-         /* retire finished batches */
-         for ( auto& b : s->batch )
+         /* retire finished batches in launch order (the older of two busy
+          * batches is on next_handle) */
+         for ( int i = 0; i < 2; ++i )
          {
-            if ( b.busy && hipEventQuery(b.done) == hipSuccess )
-            {
-               retire_batch(s, b);
-               did = true;
-            }
+            Batch& b = s->batch[(s->next_handle + i) % 2];
+            if ( !b.busy ) continue;
+            if ( hipEventQuery(b.done) != hipSuccess ) break;
+            retire_batch(s, b);
+            did = true;
          }
          /* finish copied queries */
          std::vector<std::shared_ptr<Query>> copying;
@@ -373,26 +407,38 @@ static void scheduler(kinc_server* s)
             did = true;
             s->cv.notify_all();
          }
-         /* admit queued queries in arrival order while their buffers fit */
+         /* admit queued queries in arrival order, each one whose buffer
+          * fits; one that does not fit holds back the queries behind it only
+          * once it has waited longer than the aging limit */
          {
             std::lock_guard<std::mutex> g(s->mu);
             if ( s->stop ) break;
-            while ( !s->pending.empty() )
+            const auto now = Clock::now();
+            for ( auto it = s->pending.begin(); it != s->pending.end(); )
             {
-               auto& q = s->pending.front();
+               auto q = *it;
                size_t off = 0;
-               if ( !s->arena->alloc(q->bytes, &off) ) break;
-               q->off = off;
-               q->state = 1;
-               s->active.push_back(q);
-               s->pending.pop_front();
-               did = true;
+               if ( s->arena->alloc(q->bytes, &off) )
+               {
+                  q->off = off;
+                  q->state = 1;
+                  s->active.push_back(q);
+                  it = s->pending.erase(it);
+                  did = true;
+                  continue;
+               }
+               if ( std::chrono::duration<double>(now - q->t_submit).count() > kAgingSeconds ) break;
+               ++it;
             }
          }
-         /* keep two batches in flight */
-         for ( auto& b : s->batch )
+         /* keep a batch in flight on each handle, the handles in turn */
+         for ( int i = 0; i < 2; ++i )
          {
-            if ( !b.busy && launch_batch(s, b) ) did = true;
+            Batch& b = s->batch[s->next_handle];
+            if ( b.busy ) break;
+            if ( !launch_batch(s, b) ) break;
+            s->next_handle ^= 1;
+            did = true;
          }
          if ( !did )
          {
@@ -432,8 +478,16 @@ extern "C" int kinc_server_create(const kinc_sim_params* params, const float* em
    s->S = S;
    s->MC = p.max_clusters;
    s->cap = batch_pairs;
-   int rc = kinc_sim_create(&p, emx, G, S, device, &s->sim);
-   if ( rc != KINC_OK ) return server_fail(nullptr, rc, kinc_sim_last_error(nullptr));
+   for ( int i = 0; i < 2; ++i )
+   {
+      int rc = kinc_sim_create(&p, emx, G, S, device, &s->sim[i]);
+      if ( rc != KINC_OK )
+      {
+         const std::string m = kinc_sim_last_error(nullptr);
+         kinc_server_destroy(s.release());
+         return server_fail(nullptr, rc, m);
+      }
+   }
    auto cleanup_fail = [&](const std::string& m) {
       g_server_create_error = m;
       kinc_server_destroy(s.release());
@@ -446,13 +500,18 @@ extern "C" int kinc_server_create(const kinc_sim_params* params, const float* em
       if ( e_ != hipSuccess ) return cleanup_fail(std::string(#call) + ": " + hipGetErrorString(e_)); \
    } while ( 0 )
    CR(hipSetDevice(device));
-   CR(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+   for ( int i = 0; i < 2; ++i ) CR(hipStreamCreateWithFlags(&s->stream[i], hipStreamNonBlocking));
    CR(hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking));
+   CR(hipEventCreate(&s->t0));
+   CR(hipEventRecord(s->t0, s->stream[0]));
+   CR(hipEventSynchronize(s->t0));
    CR(hipMalloc(&s->d_arena, (size_t)hbm_budget_bytes));
    s->budget = (size_t)hbm_budget_bytes & ~(size_t)255;
    s->arena.reset(new Arena((size_t)hbm_budget_bytes));
-   for ( auto& b : s->batch )
+   for ( int i = 0; i < 2; ++i )
    {
+      Batch& b = s->batch[i];
+      b.handle = i;
       CR(hipHostMalloc(&b.h_pairs, sizeof(int32_t) * 2 * (size_t)batch_pairs, 0));
       CR(hipHostMalloc(&b.h_seg, sizeof(int32_t) * (size_t)batch_pairs, 0));
       CR(hipMalloc(&b.d_pairs, sizeof(int32_t) * 2 * (size_t)batch_pairs));
@@ -541,6 +600,20 @@ extern "C" int kinc_server_result(kinc_server* s, int64_t id, kinc_hit* hits, in
    return KINC_OK;
 }
 
+extern "C" int kinc_server_timing(kinc_server* s, int64_t id, double* start_wait_s, double* run_s)
+{
+   if ( !s ) return KINC_E_INVALID;
+   std::lock_guard<std::mutex> g(s->mu);
+   auto it = s->table.find(id);
+   if ( it == s->table.end() ) return KINC_E_INVALID;
+   auto q = it->second;
+   if ( q->state != 3 ) return KINC_E_INVALID;
+   const auto first = q->started ? q->t_first : q->t_done;
+   if ( start_wait_s ) *start_wait_s = std::chrono::duration<double>(first - q->t_submit).count();
+   if ( run_s ) *run_s = std::chrono::duration<double>(q->t_done - first).count();
+   return KINC_OK;
+}
+
 extern "C" int kinc_server_stats(kinc_server* s, int64_t* pairs_done, int64_t* batches, double* busy_s)
 {
    if ( !s ) return KINC_E_INVALID;
@@ -561,7 +634,11 @@ extern "C" void kinc_server_destroy(kinc_server* s)
    }
    if ( s->worker.joinable() ) s->worker.join();
    (void)hipSetDevice(s->device);
-   if ( s->stream ) (void)hipStreamSynchronize(s->stream);
+   for ( int i = 0; i < 2; ++i )
+   {
+      if ( s->stream[i] ) (void)hipStreamSynchronize(s->stream[i]);
+      if ( s->sim[i] ) (void)kinc_sim_synchronize(s->sim[i]);
+   }
    if ( s->copy ) (void)hipStreamSynchronize(s->copy);
    for ( auto& b : s->batch )
    {
@@ -579,9 +656,13 @@ extern "C" void kinc_server_destroy(kinc_server* s)
       if ( kv.second->copied ) (void)hipEventDestroy(kv.second->copied);
    }
    (void)hipFree(s->d_arena);
-   if ( s->stream ) (void)hipStreamDestroy(s->stream);
+   for ( int i = 0; i < 2; ++i )
+   {
+      if ( s->stream[i] ) (void)hipStreamDestroy(s->stream[i]);
+      kinc_sim_destroy(s->sim[i]);
+   }
    if ( s->copy ) (void)hipStreamDestroy(s->copy);
-   kinc_sim_destroy(s->sim);
+   if ( s->t0 ) (void)hipEventDestroy(s->t0);
    delete s;
 }
 

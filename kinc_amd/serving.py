@@ -23,6 +23,7 @@ SIGNATURES = {
     "kinc_server_wait": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double]),
     "kinc_server_result": (ctypes.c_int, [_P, ctypes.c_int64, _P, ctypes.c_int64, _P, _P, _P]),
     "kinc_server_stats": (ctypes.c_int, [_P, _P, _P, _P]),
+    "kinc_server_timing": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P]),
     "kinc_server_destroy": (None, [_P]),
     "kinc_server_last_error": (ctypes.c_char_p, [_P]),
 }
@@ -99,6 +100,14 @@ class Server:
         self._check(l.kinc_server_result(self.h, qid, out.ctypes.data, n.value, ctypes.byref(n),
                                          ctypes.byref(lat), ctypes.byref(npairs)))
         return out, lat.value, npairs.value
+
+    def timing(self, qid):
+        """(seconds from submit to the first block's launch, seconds from
+        there to completion) of a finished query not yet released."""
+        w = ctypes.c_double()
+        r = ctypes.c_double()
+        self._check(_bind().kinc_server_timing(self.h, qid, ctypes.byref(w), ctypes.byref(r)))
+        return w.value, r.value
 
     def stats(self):
         pairs = ctypes.c_int64()

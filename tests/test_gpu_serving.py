@@ -124,3 +124,43 @@ def test_server_admission_under_small_budget():
         srv.close()
     want = _oracle_hits(p, x, queries[1], np.float32(0.3))
     assert _same_hits(res[1][0], want)
+
+
+def test_server_small_query_overtakes_large_ones():
+    """Fewest remaining blocks first, one-block quanta while several queries
+    have work, and admission that a query which does not fit cannot block:
+    a 1-gene query submitted behind two large ones (the second of which does
+    not fit in the budget until the first is done) finishes first, and every
+    answer equals the query's answer alone."""
+    import time
+    x = gemgen.modules(3000, 160, seed=24, nan_rate=0.08)
+    p = _native.make_params(clusmethod="gmm", corrmethod="pearson", crit="BIC")
+    big1 = list(range(0, 150))       # about 439k pairs: 14 work blocks
+    big2 = list(range(150, 300))
+    small = [2500]
+    # 21 result bytes per pair: one large query and the small one fit, not two large
+    srv = serving.Server(x, p, hbm_budget=12 << 20, batch_pairs=32768)
+    try:
+        alone = {}
+        for name, genes in (("big1", big1), ("big2", big2), ("small", small)):
+            qid = srv.submit(genes, 0.5)
+            assert srv.wait(qid, 300)
+            alone[name] = srv.result(qid)
+        t0 = time.perf_counter()
+        ids = {"big1": srv.submit(big1, 0.5), "big2": srv.submit(big2, 0.5)}
+        time.sleep(0.05)
+        ids["small"] = srv.submit(small, 0.5)
+        offs = {"big1": 0.0, "big2": 0.0, "small": time.perf_counter() - t0}
+        done = {}
+        for name, qid in ids.items():
+            assert srv.wait(qid, 300)
+            wait_s, run_s = srv.timing(qid)
+            hits, lat, npairs = srv.result(qid)
+            done[name] = offs[name] + lat
+            print(name, "latency %.3f s, start wait %.3f s, run %.3f s" % (lat, wait_s, run_s),
+                  flush=True)
+            assert npairs == alone[name][2]
+            assert _same_hits(hits, alone[name][0]), name
+        assert done["small"] < done["big1"] < done["big2"]
+    finally:
+        srv.close()
