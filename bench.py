@@ -259,6 +259,18 @@ def cpu_baseline(oracle, native, x, flags, blocks, seconds, gpu_out):
     return done / t_used, done, t_used, done, mismatches, fp64
 
 
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name")."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def host_threads():
     """Host threads the all-core CPU leg may use: the affinity set, at most 16
     (a one-GPU share of the host)."""
@@ -399,6 +411,108 @@ def serve_leg(native, gemgen, n_queries, qps, max_genes, big_query, seed=11):
     return out
 
 
+def graph_leg(native, gemgen, torch, reps=20):
+    """Config A (1,000 x 200, Pearson, no clustering: 16 work blocks of a
+    few launches each, so launch gaps matter most) on one handle, eager
+    (every block's kernels enqueued by the host) against the same blocks
+    recorded once into a HIP graph and replayed; best of `reps` passes over
+    all blocks each, outputs checked equal after every replay."""
+    x = gemgen.make("A")
+    G, S = x.shape
+    cfg = gemgen.CONFIGS["A"]
+    p = native.make_params(**cfg["params"])
+    total = native.pair_count(G)
+    starts = list(range(0, total, BLOCK))
+    n = len(starts)
+    dev = torch.device("cuda:0")
+    MC = p.max_clusters
+
+    def outputs():
+        return (torch.empty((n, BLOCK), dtype=torch.int8, device=dev),
+                torch.empty((n, BLOCK, S), dtype=torch.int8, device=dev),
+                torch.empty((n, BLOCK, MC), dtype=torch.float32, device=dev),
+                torch.empty((n, BLOCK, MC), dtype=torch.int16, device=dev))
+
+    eng = native.Engine(x, p)
+    try:
+        out = outputs()
+
+        def eager_pass():
+            for i, st in enumerate(starts):
+                eng.execute_device(st, min(BLOCK, total - st), *(t[i].data_ptr() for t in out))
+
+        # the same fill before the eager pass and before every replay: rows
+        # past the last block's pairs are written by neither
+        for t in out:
+            t.fill_(-3)
+        eager_pass()
+        eng.synchronize()
+        want = [t.clone() for t in out]
+        best_eager = 1e9
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            eager_pass()
+            eng.synchronize()
+            best_eager = min(best_eager, time.perf_counter() - t0)
+        eng.graph_begin()
+        eager_pass()
+        g = eng.graph_end()
+        best_graph = 1e9
+        same = True
+        for _ in range(reps):
+            for t in out:
+                t.fill_(-3)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.launch(None)
+            torch.cuda.synchronize()
+            best_graph = min(best_graph, time.perf_counter() - t0)
+            # bytes, so that NaN correlations compare equal to themselves
+            same = same and all(torch.equal(a.view(torch.uint8), b.view(torch.uint8))
+                                for a, b in zip(out, want))
+        g.close()
+        host = [tuple(a[i, :min(BLOCK, total - st)].cpu().numpy() for a in want)
+                for i, st in enumerate(starts)]
+    finally:
+        eng.close()
+    res = {"workload": WORKLOADS["A"], "blocks": n, "pairs": int(total),
+           "eager_pairs_per_s": total / best_eager, "graph_pairs_per_s": total / best_graph,
+           "eager_ms": best_eager * 1e3, "graph_ms": best_graph * 1e3,
+           "replays_identical": bool(same), "handles": 1}
+    log("config A on one handle: eager %.3f ms, graph %.3f ms per pass over %d blocks; "
+        "replays identical: %s" % (best_eager * 1e3, best_graph * 1e3, n, same))
+    res["_device_outputs"] = (x, p, starts, host)
+    return res
+
+
+def cpu_baseline_config_a(oracle, graph_obj):
+    """Config A timed in full on the CPU oracle (one thread, then one thread
+    per work block), every pair checked against the device outputs of the
+    graph leg."""
+    from concurrent.futures import ThreadPoolExecutor
+    from parity_util import count_mismatching_pairs
+    x, p, starts, host = graph_obj["_device_outputs"]
+    total = graph_obj["pairs"]
+    t0 = time.perf_counter()
+    cpu = [oracle.execute_block(p, x, st, min(BLOCK, total - st)) for st in starts]
+    t1 = time.perf_counter() - t0
+    threads = host_threads()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        cpu_t = list(ex.map(lambda st: oracle.execute_block(p, x, st, min(BLOCK, total - st)),
+                            starts))
+    tn = time.perf_counter() - t0
+    bad = sum(count_mismatching_pairs((h[0], h[1], h[2], h[3].view(np.uint16)), c)
+              for h, c in zip(host, cpu))
+    bad += sum(count_mismatching_pairs(a, b) for a, b in zip(cpu, cpu_t))
+    log("config A in full on the CPU oracle: %.0f pairs/s on 1 thread, %.0f on %d; %d "
+        "mismatching pairs" % (total / t1, total / tn, threads, bad))
+    return {"workload": WORKLOADS["A"], "pairs": int(total), "kind": "port",
+            "single_thread_pairs_per_s": total / t1, "threads": threads,
+            "threads_pairs_per_s": total / tn, "mismatching_pairs": int(bad),
+            "cpu_model": cpu_model(), "host_cores": os.cpu_count()}
+
+
 def rmt_leg(gemgen, with_cpu):
     """The RMT threshold sweep (include/kinc_rmt.h) on a 5,000-gene matrix
     of 1,000 independent pairs and a 3,000-gene correlated block, and on the
@@ -499,6 +613,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the handles' default launch configuration (no kinc_sim_autotune)")
     ap.add_argument("--no-rmt", action="store_true", help="skip the RMT threshold-sweep leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the config A eager-versus-HIP-graph leg")
     ap.add_argument("--serve-queries", type=int, default=8)
     ap.add_argument("--serve-qps", type=float, default=0.1)
     ap.add_argument("--serve-max-genes", type=int, default=500)
@@ -643,6 +759,10 @@ def main():
     profiled = (args.config == "H" and not args.samples and not args.clusmethod
                 and not args.pearson_fp64 and not args.set)
 
+    graph_obj = None
+    if not args.no_graph:
+        graph_obj = graph_leg(native, gemgen, torch)
+
     cpu = None
     fp64_err = None
     if not args.no_cpu:
@@ -657,6 +777,7 @@ def main():
         log("cpu oracle: %d pairs in %.1fs cpu -> %.2f pairs/s; parity %d/%d mismatching"
             % (n, used, rate, bad, checked))
         cpu = {"value": rate, "unit": "pairs/s", "cores": 1, "kind": "port",
+               "cpu_model": cpu_model(), "host_cores": os.cpu_count(),
                "sample": "%d pairs: a prefix of each of the %d timed work blocks, single-threaded "
                          "C oracle (-O2, no FP contraction); GPU output bit-exact on %d/%d"
                          % (n, len(blocks), checked - bad, checked)}
@@ -668,10 +789,15 @@ def main():
                 log("cpu oracle, %d threads: %d pairs in %.1fs wall -> %.2f pairs/s; parity %d/%d "
                     "mismatching" % (used_t, mn, mwall, mrate, mbad, mn))
                 cpu = {"value": mrate, "unit": "pairs/s", "cores": used_t, "kind": "port",
+                       "cpu_model": cpu_model(), "host_cores": os.cpu_count(),
                        "sample": "%d pairs: one run of %d pairs of a timed work block per thread, "
                                  "C oracle (-O2, no FP contraction) on %d host threads; GPU output "
                                  "bit-exact on %d/%d" % (mn, mn // used_t, used_t, mn - mbad, mn),
                        "single_thread": {"value": rate, "cores": 1, "sample": cpu["sample"]}}
+        if graph_obj is not None:
+            cpu["config_a_full"] = cpu_baseline_config_a(oracle, graph_obj)
+    if graph_obj is not None:
+        graph_obj.pop("_device_outputs", None)
 
     rmt_obj = None
     if not args.no_rmt:
@@ -726,6 +852,7 @@ def main():
         "cpu_baseline": cpu,
         "corr_error_vs_fp64": fp64_err,
         "serving": serving_obj,
+        "graph": graph_obj,
         "rmt": rmt_obj,
     }
     print(json.dumps(line), flush=True)

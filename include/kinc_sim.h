@@ -205,6 +205,24 @@ int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc_sim_tuning
 int kinc_sim_get_tuning(const kinc_sim* h, kinc_sim_tuning* out);
 int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t);
 
+/*
+ * HIP graphs of work blocks (no counterpart in the reference, whose OpenCL
+ * worker enqueues every kernel of every sub-batch itself).  Between
+ * kinc_sim_graph_begin and kinc_sim_graph_end, kinc_sim_execute_device and
+ * kinc_sim_execute_pairs_device calls on the handle's own stream (stream
+ * NULL) are recorded instead of run; kinc_sim_graph_end instantiates them as
+ * one graph, which kinc_sim_graph_launch replays on `stream` with the same
+ * pair ranges and output buffers (the device queues are cleared by kernel
+ * nodes, so every replay gives the results of the first).  Host-buffer
+ * calls and kernel timing are refused while recording.  A graph outlives
+ * neither its handle nor its buffers.
+ */
+typedef struct kinc_sim_graph kinc_sim_graph;
+int kinc_sim_graph_begin(kinc_sim* h);
+int kinc_sim_graph_end(kinc_sim* h, kinc_sim_graph** graph);
+int kinc_sim_graph_launch(kinc_sim_graph* graph, void* stream);
+void kinc_sim_graph_destroy(kinc_sim_graph* graph);
+
 /* Number of linear pairs for G genes: G(G-1)/2 (src/core/similarity.cpp:19). */
 int64_t kinc_pair_count(int32_t gene_size);
 

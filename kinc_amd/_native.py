@@ -74,6 +74,10 @@ SIGNATURES = [
     ("kinc_sim_autotune", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P]),
     ("kinc_sim_get_tuning", ctypes.c_int, [_P, _P]),
     ("kinc_sim_set_tuning", ctypes.c_int, [_P, _P]),
+    ("kinc_sim_graph_begin", ctypes.c_int, [_P]),
+    ("kinc_sim_graph_end", ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    ("kinc_sim_graph_launch", ctypes.c_int, [_P, _P]),
+    ("kinc_sim_graph_destroy", None, [_P]),
     ("kinc_pair_count", ctypes.c_int64, [ctypes.c_int32]),
     ("kinc_pair_index", None, [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
                                ctypes.POINTER(ctypes.c_int32)]),
@@ -228,6 +232,17 @@ class Engine:
     def synchronize(self):
         self._check(lib().kinc_sim_synchronize(self.h))
 
+    def graph_begin(self):
+        """Record the following execute_device calls (stream None) into a HIP
+        graph instead of running them (kinc_sim_graph_begin)."""
+        self._check(lib().kinc_sim_graph_begin(self.h))
+
+    def graph_end(self):
+        """Finish recording; returns a Graph that replays the recorded work."""
+        g = _P()
+        self._check(lib().kinc_sim_graph_end(self.h, ctypes.byref(g)))
+        return Graph(g)
+
     def autotune(self, start, count):
         """Time the handle's launch configurations on pairs [start, start+count)
         and keep the fastest (kinc_sim_autotune); returns it as a dict."""
@@ -250,6 +265,29 @@ class Engine:
         if getattr(self, "h", None):
             lib().kinc_sim_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A recorded sequence of work blocks (kinc_sim_graph)."""
+
+    def __init__(self, g):
+        self.g = g
+
+    def launch(self, stream=None):
+        rc = lib().kinc_sim_graph_launch(self.g, stream)
+        if rc != KINC_OK:
+            raise KincError(rc, "kinc_sim_graph_launch failed")
+
+    def close(self):
+        if getattr(self, "g", None):
+            lib().kinc_sim_graph_destroy(self.g)
+            self.g = None
 
     def __del__(self):
         try:

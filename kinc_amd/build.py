@@ -46,7 +46,7 @@ SOURCES = [
 # memory-store instance used above the register range (1024 samples for up to
 # 5 clusters, 512 for up to 8)
 INSTANCES = ([(5, p) for p in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16)]
-             + [(8, p) for p in (0, 2, 4, 8)])
+             + [(8, p) for p in (0, 2, 4, 8)] + [(64, 0)])
 # the instances of at most 6 slots (N <= 384) are built a second time for 5
 # waves per SIMD (96 VGPRs); the library uses those when the per-wave LDS
 # leaves room for 20 waves per CU (about 400 samples or fewer): 1.11 M

@@ -26,7 +26,7 @@
 
 #define KINC_FOR_EACH_INSTANCE(X)                                                   \
    X(5, 0) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(5, 10)   \
-   X(5, 12) X(5, 14) X(5, 16) X(8, 0) X(8, 2) X(8, 4) X(8, 8)
+   X(5, 12) X(5, 14) X(5, 16) X(8, 0) X(8, 2) X(8, 4) X(8, 8) X(64, 0)
 /* small instances also built for 5 waves per SIMD */
 #define KINC_FOR_EACH_INSTANCE_W5(X) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(8, 2) X(8, 4)
 
@@ -61,6 +61,12 @@ thread_local char g_create_error[256];
  * kMaxBins classes to out and returns their number. */
 int slot_classes(int S, int kmax, int* out)
 {
+   if ( kmax > 8 )
+   {
+      /* max_clusters 9..64: the run-time-K memory-store instance */
+      out[0] = 0;
+      return 1;
+   }
    const int* list = kmax == 5 ? kSlotList5 : kSlotList8;
    const int n = kmax == 5 ? 12 : 3;
    if ( S > 64 * list[n - 1] )
@@ -150,6 +156,7 @@ struct kinc_sim
    char err[256] = {};
    /* kernel timing */
    bool profiling = false;
+   bool capturing = false;         /* between kinc_sim_graph_begin and _end */
    /* the per-wave LDS leaves room for 20 waves per CU: small instances run
     * their 5-waves-per-SIMD builds */
    bool w5 = false;
@@ -254,11 +261,6 @@ static int validate(const kinc_sim_params* p, int G, int S, const char** why)
       if ( r != 0 ) { *why = "reserved fields must be zero"; return KINC_E_INVALID; }
    }
    if ( G < 2 || S < 1 ) { *why = "need at least 2 genes and 1 sample"; return KINC_E_INVALID; }
-   if ( p->max_clusters > 8 )
-   {
-      *why = "max_clusters > 8 is not built on the device path";
-      return KINC_E_UNSUPPORTED;
-   }
    if ( S > KINC_MAX_SAMPLES )
    {
       *why = "sample_size > 32768 is not supported";
@@ -277,7 +279,7 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
    h->p = *params;
    h->G = G;
    h->S = S;
-   h->kmax = params->max_clusters <= 5 ? 5 : 8;
+   h->kmax = params->max_clusters <= 5 ? 5 : (params->max_clusters <= 8 ? 8 : 64);
    h->cap = params->batch_capacity > 0 ? params->batch_capacity : KINC_WORK_BLOCK_SIZE;
 
    /* bins: every register-slot count up to the one S needs, or the single
@@ -337,13 +339,24 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       a.lds_u = (int32_t)off;
       off = align16(off + sort_bytes);
       a.scr_lg = (int32_t)off;
-      off = align16(off + sizeof(float) * (size_t)(h->kmax + 1) * (size_t)a.mem_slots * 64);
+      /* per-point responsibilities and log p(x); the run-time-K fits keep
+       * log p(x) only and recompute the rest */
+      const size_t lg_rows = h->kmax > 8 ? 1 : (size_t)(h->kmax + 1);
+      off = align16(off + sizeof(float) * lg_rows * (size_t)a.mem_slots * 64);
       a.scr_y = (int32_t)off;
       off = align16(off + (size_t)2 * (size_t)a.mem_slots * 64);
       a.scratch_stride = (int64_t)((off + 255) & ~(size_t)255);
       a.lds_prm = 0;
-      a.lds_tile = (int32_t)align16(sizeof(float) * 8 * 8);
-      h->lds_bytes = align16((size_t)a.lds_tile + tile_bytes);
+      if ( h->kmax > 8 )
+      {
+         a.lds_tile = (int32_t)align16(KINC_DYN_PRM_BYTES);
+         h->lds_bytes = align16((size_t)a.lds_tile + sizeof(float) * KINC_DYN_TILE_ROWS * (size_t)a.tile_stride);
+      }
+      else
+      {
+         a.lds_tile = (int32_t)align16(sizeof(float) * 8 * 8);
+         h->lds_bytes = align16((size_t)a.lds_tile + tile_bytes);
+      }
       h->w5 = false;
    }
 }
@@ -495,6 +508,7 @@ static int launch_pairs(kinc_sim* h, int P, const void* fn, unsigned grid, void*
 extern "C" int kinc_sim_set_profiling(kinc_sim* h, int32_t enabled)
 {
    if ( !h ) return KINC_E_INVALID;
+   if ( h->capturing && enabled ) return fail_c(h, KINC_E_INVALID, "no kernel timing while a graph is recorded");
    h->profiling = enabled != 0;
    return KINC_OK;
 }
@@ -701,6 +715,10 @@ extern "C" int kinc_sim_execute_device(kinc_sim* h, int64_t start, int64_t count
    if ( start < 0 || start + count > kinc_pair_count(h->G) ) return fail(h, KINC_E_INVALID, "pair range outside the matrix");
    if ( count == 0 ) return KINC_OK;
    if ( !K || !corr ) return fail(h, KINC_E_INVALID, "K and corr outputs are required");
+   if ( h->capturing && stream && (hipStream_t)stream != h->stream )
+   {
+      return fail_c(h, KINC_E_INVALID, "while a graph is recorded, work goes on the handle's own stream");
+   }
    HIP_TRY(h, hipSetDevice(h->device));
    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
    return run(h, start, nullptr, nullptr, (int)count, K, labels, corr, iters, s);
@@ -714,6 +732,10 @@ extern "C" int kinc_sim_execute_pairs_device(kinc_sim* h, const int32_t* pairs, 
    if ( count < 0 || count > h->cap ) return fail(h, KINC_E_INVALID, "count must lie in [0, batch_capacity]");
    if ( count == 0 ) return KINC_OK;
    if ( !pairs || !K || !corr ) return fail(h, KINC_E_INVALID, "pairs, K and corr are required");
+   if ( h->capturing && stream && (hipStream_t)stream != h->stream )
+   {
+      return fail_c(h, KINC_E_INVALID, "while a graph is recorded, work goes on the handle's own stream");
+   }
    HIP_TRY(h, hipSetDevice(h->device));
    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
    return run(h, 0, pairs, seg_lo, (int)count, K, labels, corr, iters, s);
@@ -723,6 +745,7 @@ extern "C" int kinc_sim_execute_pairs(kinc_sim* h, const int32_t* pairs, const i
                                       int64_t count, int8_t* K, int8_t* labels, float* corr,
                                       uint16_t* iters)
 {
+   if ( h && h->capturing ) return fail_c(h, KINC_E_INVALID, "host-buffer calls cannot be recorded into a graph");
    if ( !h ) return KINC_E_INVALID;
    if ( count < 0 || count > h->cap ) return fail(h, KINC_E_INVALID, "count must lie in [0, batch_capacity]");
    if ( count == 0 ) return KINC_OK;
@@ -754,6 +777,7 @@ extern "C" int kinc_sim_execute_pairs(kinc_sim* h, const int32_t* pairs, const i
 extern "C" int kinc_sim_execute(kinc_sim* h, int64_t start, int64_t count,
                                 int8_t* K, int8_t* labels, float* corr, uint16_t* iters)
 {
+   if ( h && h->capturing ) return fail_c(h, KINC_E_INVALID, "host-buffer calls cannot be recorded into a graph");
    if ( !h ) return KINC_E_INVALID;
    if ( count == 0 ) return KINC_OK;
    if ( !K || !corr ) return fail(h, KINC_E_INVALID, "K and corr outputs are required");
@@ -946,4 +970,60 @@ extern "C" int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t)
    h->w5 = t->w5 != 0;
    h->naux = t->aux_streams;
    return KINC_OK;
+}
+
+/* ------------------------------------------------------------ HIP graphs */
+
+struct kinc_sim_graph
+{
+   hipGraph_t graph = nullptr;
+   hipGraphExec_t exec = nullptr;
+   int device = 0;
+};
+
+extern "C" int kinc_sim_graph_begin(kinc_sim* h)
+{
+   if ( !h ) return KINC_E_INVALID;
+   if ( h->capturing ) return fail_c(h, KINC_E_INVALID, "a graph is already being recorded");
+   if ( h->profiling ) return fail_c(h, KINC_E_INVALID, "no graph recording while kernel timing is on");
+   HIP_TRY(h, hipSetDevice(h->device));
+   HIP_TRY(h, hipStreamSynchronize(h->stream));
+   HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+   h->capturing = true;
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_graph_end(kinc_sim* h, kinc_sim_graph** out)
+{
+   if ( !h || !out ) return KINC_E_INVALID;
+   if ( !h->capturing ) return fail_c(h, KINC_E_INVALID, "no graph is being recorded");
+   *out = nullptr;
+   h->capturing = false;
+   kinc_sim_graph* g = new kinc_sim_graph();
+   g->device = h->device;
+   hipError_t e = hipStreamEndCapture(h->stream, &g->graph);
+   if ( e == hipSuccess ) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+   if ( e != hipSuccess )
+   {
+      kinc_sim_graph_destroy(g);
+      return fail_hip(h, "recording the graph", e);
+   }
+   *out = g;
+   return KINC_OK;
+}
+
+extern "C" int kinc_sim_graph_launch(kinc_sim_graph* g, void* stream)
+{
+   if ( !g || !g->exec ) return KINC_E_INVALID;
+   if ( hipSetDevice(g->device) != hipSuccess ) return KINC_E_DEVICE;
+   return hipGraphLaunch(g->exec, (hipStream_t)stream) == hipSuccess ? KINC_OK : KINC_E_DEVICE;
+}
+
+extern "C" void kinc_sim_graph_destroy(kinc_sim_graph* g)
+{
+   if ( !g ) return;
+   (void)hipSetDevice(g->device);
+   if ( g->exec ) (void)hipGraphExecDestroy(g->exec);
+   if ( g->graph ) (void)hipGraphDestroy(g->graph);
+   delete g;
 }

@@ -24,6 +24,12 @@
  * offset field. */
 #define KINC_TILE_STRIDE 68
 
+/* The run-time-K instance (max_clusters 9..64, KMAX = 64, memory store):
+ * LDS bytes of its parameter area (two sets of 64 components x 8 floats and
+ * 64 per-component maxima) and rows of its chain tile. */
+#define KINC_DYN_PRM_BYTES (4 * (128 * 8 + 64))
+#define KINC_DYN_TILE_ROWS 64
+
 /* Bins of the persistent form: at most this many register-slot classes. */
 #define KINC_MAX_BINS 12
 

@@ -1572,6 +1572,8 @@ __device__ __forceinline__ float spearman_k(const SimArgs& a, const Lds& L, int 
    return (float)(1.0 - 6.0 * (double)(int32_t)d / (double)denom);
 }
 
+#include "sim_pair_dyn.h"
+
 /* -------------------------------------------------------------- driver */
 
 /* One clustering fit with K components; P > 0 selects the register store
@@ -1654,7 +1656,16 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
          float bestValue = INFINITY;
          for ( int Kc = a.min_k; Kc <= a.max_k; ++Kc )
          {
-            FitOut f = fit_dispatch<P, KMAX>(a, L, N, Kc);
+            FitOut f;
+            if constexpr ( KMAX > 8 )
+            {
+               /* run-time K (memory store): max_clusters 9..64 */
+               f = (a.clus == 1) ? gmm_fit_dyn(L, N, Kc, a.crit == 1) : kmeans_fit_dyn(a, L, N, Kc);
+            }
+            else
+            {
+               f = fit_dispatch<P, KMAX>(a, L, N, Kc);
+            }
             if ( lane == Kc - 1 ) itv = f.iters;
             if ( !f.ok ) continue;
 
@@ -1682,6 +1693,45 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
       K = bestK;
    }
 
+   if constexpr ( KMAX > 8 )
+   {
+      /* up to 64 correlations: kept in LDS (the parameter area, free after
+       * the fits), written out below */
+      float* cl = L.prm;
+      if ( K > 0 )
+      {
+         if ( a.corr == 0 )
+         {
+            if ( N < S ) fill_stale(a, L, q, N);
+            for ( int k0 = 0; k0 < K; k0 += 12 )
+            {
+               pearson_group(a, L, k0, K - k0 < 12 ? K - k0 : 12, cl);
+            }
+         }
+         else
+         {
+            for ( int k = 0; k < K; ++k )
+            {
+               const float r = spearman_k(a, L, k);
+               if ( lane == 0 ) cl[k] = r;
+            }
+         }
+      }
+      wave_sync(true);
+      if ( lane == 0 ) a.K_out[q] = (int8_t)K;
+      float* co = a.corr_out + (size_t)q * MC;
+      if ( lane < MC ) co[lane] = lane < K ? cl[lane] : NAN;
+      if ( a.iters_out && lane < MC ) a.iters_out[(size_t)q * MC + lane] = itv;
+      if ( a.labels_out )
+      {
+         int8_t* lo = a.labels_out + (size_t)q * S;
+         for ( int s = lane; s < S; s += WAVE ) lo[s] = L.lab[s];
+      }
+      wave_sync(true);
+      return;
+   }
+   else
+   {
    float corr[KMAX];
 #pragma unroll
    for ( int k = 0; k < KMAX; ++k ) corr[k] = NAN;
@@ -1716,6 +1766,7 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
       for ( int s = lane; s < S; s += WAVE ) lo[s] = L.lab[s];
    }
    wave_sync(L.lgbuf != nullptr);
+   }
 }
 
 } // namespace

@@ -65,7 +65,6 @@ def test_pair_index_abi(native):
     (dict(minclus=3, maxclus=2), _native.KINC_E_INVALID),
     (dict(minsamp=0), _native.KINC_E_INVALID),
     (dict(maxclus=65), _native.KINC_E_INVALID),
-    (dict(maxclus=9), _native.KINC_E_UNSUPPORTED),
 ])
 def test_create_rejects_bad_params_before_device(native, bad, code):
     import numpy as np
