@@ -573,16 +573,17 @@ def valu_calibration():
     lds = 160 * 1024 // 16
     iters = 20000
     rows = []
-    for mode, (chains, lanes) in enumerate(native.VALU_PROBE_MODES):
+    for mode, (chains, lanes, op) in enumerate(native.VALU_PROBE_MODES):
         ms = native.valu_probe(mode, iters, wgs, lds)
         insts = 128.0 * iters * wgs
         rate = insts / (ms * 1e-3)
         cpi = (cus * 4) * CLOCK_HZ / rate
-        rows.append({"mode": mode, "chains": chains, "lanes": lanes, "ms": ms,
+        name = native.VALU_PROBE_OPS[op]
+        rows.append({"mode": mode, "op": name, "chains": chains, "lanes": lanes, "ms": ms,
                      "wave_insts": insts, "insts_per_s": rate,
                      "cycles_per_inst_per_simd": cpi})
-        log("valu probe mode %d (%d chains, %d lanes): %.3f ms, %.3g wave insts/s, "
-            "%.2f cycles per instruction per SIMD" % (mode, chains, lanes, ms, rate, cpi))
+        log("valu probe mode %d (%s, %d chains, %d lanes): %.3f ms, %.3g wave insts/s, "
+            "%.2f cycles per instruction per SIMD" % (mode, name, chains, lanes, ms, rate, cpi))
     print(json.dumps({"valu_calibration": rows, "cus": cus, "workgroups": wgs,
                       "iters": iters, "clock_hz": CLOCK_HZ}), flush=True)
 

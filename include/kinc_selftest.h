@@ -25,10 +25,14 @@ int kinc_selftest_math(int32_t fn, const float* in, float* out, int64_t n);
 /*
  * VALU issue-rate probe: launches `workgroups` one-wave workgroups (with
  * lds_bytes of dynamic LDS each, which caps the waves resident per CU), each
- * issuing exactly 128 * iters v_add_f32 on the active lanes of `mode`:
- *   0: 8 independent chains, 64 lanes   1: 8 chains, 16 lanes
- *   2: 8 chains, 4 lanes                3: 1 dependent chain, 64 lanes
- *   4: 1 dependent chain, 16 lanes      5: 8 chains, 32 lanes
+ * issuing exactly 128 * iters instructions of one kind on the active lanes
+ * of `mode` (independent chains / active lanes / instruction):
+ *    0: 8 / 64 / v_add_f32      1: 8 / 16 / v_add_f32    2: 8 / 4 / v_add_f32
+ *    3: 1 / 64 / v_add_f32      4: 1 / 16 / v_add_f32    5: 8 / 32 / v_add_f32
+ *    6: 8 / 64 / v_mul_f32      7: 8 / 64 / v_fma_f32    8: 8 / 64 / v_fmac_f32
+ *    9: 8 / 64 / v_max_f32     10: 8 / 64 / v_rndne_f32 11: 8 / 64 / v_ldexp_f32
+ *   12: 8 / 64 / v_cvt_i32_f32 13: 8 / 64 / v_cndmask_b32
+ *   14: 8 / 64 / v_fmaak_f32   15: 1 / 64 / v_fma_f32   16: 8 / 16 / v_fma_f32
  * and writes the duration of the timed launch (after one warm-up launch) to
  * *ms.  Returns 0 on success, < 0 on error.
  */

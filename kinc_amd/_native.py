@@ -307,8 +307,12 @@ def device_math(fn, x):
     return out
 
 
-# (independent chains, active lanes) of each kinc_selftest_valu_probe mode
-VALU_PROBE_MODES = [(8, 64), (8, 16), (8, 4), (1, 64), (1, 16), (8, 32)]
+# (independent chains, active lanes, instruction) of each kinc_selftest_valu_probe mode
+VALU_PROBE_OPS = ["v_add_f32", "v_mul_f32", "v_fma_f32", "v_fmac_f32", "v_max_f32",
+                  "v_rndne_f32", "v_ldexp_f32", "v_cvt_i32_f32", "v_cndmask_b32", "v_fmaak_f32"]
+VALU_PROBE_MODES = [(8, 64, 0), (8, 16, 0), (8, 4, 0), (1, 64, 0), (1, 16, 0), (8, 32, 0),
+                    (8, 64, 1), (8, 64, 2), (8, 64, 3), (8, 64, 4), (8, 64, 5), (8, 64, 6),
+                    (8, 64, 7), (8, 64, 8), (8, 64, 9), (1, 64, 2), (8, 16, 2)]
 
 
 def valu_probe(mode, iters, workgroups, lds_bytes):

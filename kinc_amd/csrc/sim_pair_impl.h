@@ -89,16 +89,33 @@ __device__ __forceinline__ int bcast_i(int v, int src)
    return __builtin_amdgcn_readlane(v, src);
 }
 
-/* NaN-skipping max in the reference's `if (v > max) max = v` sense. */
+/* One step of a DPP maximum: v and the value of the lane that CTRL selects,
+ * in the rows of ROW_MASK (the other lanes keep v). */
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_max(float v)
+{
+   const float o = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                              CTRL, ROW_MASK, 0xF, false));
+   return (o > v) ? o : v;
+}
+
+/*
+ * NaN-skipping max in the reference's `if (v > max) max = v` sense, of
+ * values that are never NaN, so the order of comparison does not matter:
+ * quads by DPP quad permutations, then half rows and rows by mirrors, then
+ * across rows with row_bcast:15 and row_bcast:31 (lane 63 ends with the
+ * maximum), read back to every lane.  VALU moves with a few cycles of
+ * latency instead of six LDS permutes.
+ */
 __device__ __forceinline__ float wave_max(float v)
 {
-#pragma unroll
-   for ( int off = 32; off >= 1; off >>= 1 )
-   {
-      float o = __shfl_xor(v, off);
-      v = (o > v) ? o : v;
-   }
-   return v;
+   v = dpp_max<0xB1, 0xF>(v);    /* quad_perm [1,0,3,2] */
+   v = dpp_max<0x4E, 0xF>(v);    /* quad_perm [2,3,0,1] */
+   v = dpp_max<0x141, 0xF>(v);   /* row_half_mirror */
+   v = dpp_max<0x140, 0xF>(v);   /* row_mirror */
+   v = dpp_max<0x142, 0xA>(v);   /* row_bcast:15 into rows 1 and 3 */
+   v = dpp_max<0x143, 0xC>(v);   /* row_bcast:31 into rows 2 and 3 */
+   return bcast(v, 63);
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)

@@ -294,14 +294,32 @@ extern "C" int kinc_selftest_math(int32_t fn, const float* in, float* out, int64
 
 /*
  * VALU issue-rate probe (diagnostic).  Each wave runs `iters` rounds of 128
- * v_add_f32 on its first LANES lanes, spread over CHAINS independent
- * accumulators (CHAINS = 1: one dependent chain).  The issue count per wave
- * is exactly 128 * iters VALU instructions (the loop counter is scalar), so
- * the launch time gives the device's wave64 VALU issue rate for that exec
- * mask and dependency pattern, the ceiling the pair kernels are measured
- * against.
+ * instructions of one kind OP on its first LANES lanes, spread over CHAINS
+ * independent accumulators (CHAINS = 1: one dependent chain).  The issue
+ * count per wave is exactly 128 * iters VALU instructions (the loop counter
+ * is scalar), so the launch time gives the device's wave64 issue rate for
+ * that instruction, exec mask and dependency pattern: the ceilings the pair
+ * kernels' instruction mix is measured against.
+ *   OP 0 v_add_f32   1 v_mul_f32   2 v_fma_f32   3 v_fmac_f32   4 v_max_f32
+ *   5 v_rndne_f32    6 v_ldexp_f32 7 v_cvt_i32_f32 8 v_cndmask_b32
+ *   9 v_fmaak_f32 (FMA with a literal)
  */
-template <int CHAINS, int LANES>
+template <int OP>
+__device__ __forceinline__ void probe_op(float& acc, float inc)
+{
+   if constexpr ( OP == 0 ) asm volatile("v_add_f32 %0, %0, %1" : "+v"(acc) : "v"(inc));
+   if constexpr ( OP == 1 ) asm volatile("v_mul_f32 %0, %0, %1" : "+v"(acc) : "v"(inc));
+   if constexpr ( OP == 2 ) asm volatile("v_fma_f32 %0, %0, %1, %1" : "+v"(acc) : "v"(inc));
+   if constexpr ( OP == 3 ) asm volatile("v_fmac_f32 %0, %1, %1" : "+v"(acc) : "v"(inc));
+   if constexpr ( OP == 4 ) asm volatile("v_max_f32 %0, %0, %1" : "+v"(acc) : "v"(inc));
+   if constexpr ( OP == 5 ) asm volatile("v_rndne_f32 %0, %0" : "+v"(acc));
+   if constexpr ( OP == 6 ) asm volatile("v_ldexp_f32 %0, %0, %1" : "+v"(acc) : "v"(inc));
+   if constexpr ( OP == 7 ) asm volatile("v_cvt_i32_f32 %0, %0" : "+v"(acc));
+   if constexpr ( OP == 8 ) asm volatile("v_cndmask_b32 %0, %0, %1, vcc" : "+v"(acc) : "v"(inc) : "vcc");
+   if constexpr ( OP == 9 ) asm volatile("v_fmaak_f32 %0, %0, %1, 0x3f800000" : "+v"(acc) : "v"(inc));
+}
+
+template <int CHAINS, int LANES, int OP>
 __global__ __launch_bounds__(64) void k_valu_probe(float* out, int iters)
 {
    const int lane = (int)threadIdx.x;
@@ -317,10 +335,7 @@ __global__ __launch_bounds__(64) void k_valu_probe(float* out, int iters)
          for ( int r = 0; r < 128 / CHAINS; ++r )
          {
 #pragma unroll
-            for ( int c = 0; c < CHAINS; ++c )
-            {
-               asm volatile("v_add_f32 %0, %0, %1" : "+v"(acc[c]) : "v"(inc));
-            }
+            for ( int c = 0; c < CHAINS; ++c ) probe_op<OP>(acc[c], inc);
          }
       }
    }
@@ -330,27 +345,31 @@ __global__ __launch_bounds__(64) void k_valu_probe(float* out, int iters)
    out[(size_t)blockIdx.x * 64 + lane] = s;
 }
 
-/* modes of kinc_selftest_valu_probe: (independent chains, active lanes) */
-static const int kProbeModes[][2] = {{8, 64}, {8, 16}, {8, 4}, {1, 64}, {1, 16}, {8, 32}};
+/* modes of kinc_selftest_valu_probe: (independent chains, active lanes, op) */
+static const int kProbeModes[][3] = {
+   {8, 64, 0}, {8, 16, 0}, {8, 4, 0}, {1, 64, 0}, {1, 16, 0}, {8, 32, 0},
+   {8, 64, 1}, {8, 64, 2}, {8, 64, 3}, {8, 64, 4}, {8, 64, 5}, {8, 64, 6},
+   {8, 64, 7}, {8, 64, 8}, {8, 64, 9}, {1, 64, 2}, {8, 16, 2}};
+static const int kProbeCount = (int)(sizeof(kProbeModes) / sizeof(kProbeModes[0]));
+
+template <int CH, int LN, int OP>
+static const void* probe_fn() { return (const void*)&k_valu_probe<CH, LN, OP>; }
 
 extern "C" int kinc_selftest_valu_probe(int32_t mode, int32_t iters, int32_t workgroups,
                                         int32_t lds_bytes, float* ms)
 {
-   if ( mode < 0 || mode >= (int)(sizeof(kProbeModes) / sizeof(kProbeModes[0])) || iters < 1 ||
+   if ( mode < 0 || mode >= kProbeCount || iters < 1 ||
         workgroups < 1 || lds_bytes < 0 || lds_bytes > 65536 || !ms )
    {
       return -1;
    }
-   const void* fn = nullptr;
-   switch ( mode )
-   {
-   case 0: fn = (const void*)&k_valu_probe<8, 64>; break;
-   case 1: fn = (const void*)&k_valu_probe<8, 16>; break;
-   case 2: fn = (const void*)&k_valu_probe<8, 4>; break;
-   case 3: fn = (const void*)&k_valu_probe<1, 64>; break;
-   case 4: fn = (const void*)&k_valu_probe<1, 16>; break;
-   default: fn = (const void*)&k_valu_probe<8, 32>; break;
-   }
+   static const void* const fns[] = {
+      probe_fn<8, 64, 0>(), probe_fn<8, 16, 0>(), probe_fn<8, 4, 0>(), probe_fn<1, 64, 0>(),
+      probe_fn<1, 16, 0>(), probe_fn<8, 32, 0>(), probe_fn<8, 64, 1>(), probe_fn<8, 64, 2>(),
+      probe_fn<8, 64, 3>(), probe_fn<8, 64, 4>(), probe_fn<8, 64, 5>(), probe_fn<8, 64, 6>(),
+      probe_fn<8, 64, 7>(), probe_fn<8, 64, 8>(), probe_fn<8, 64, 9>(), probe_fn<1, 64, 2>(),
+      probe_fn<8, 16, 2>()};
+   const void* fn = fns[mode];
    float* d_out = nullptr;
    if ( hipMalloc(&d_out, sizeof(float) * 64 * (size_t)workgroups) != hipSuccess ) return -3;
    hipEvent_t e0 = nullptr, e1 = nullptr;
