@@ -211,6 +211,67 @@ def run_form(native, x, flags, form, blocks, warmup, torch, workers=2, autotune=
     return pairs, t1 - t0, prof, out
 
 
+def census_leg(native, x, flags, blocks, out):
+    """Phase census of the 7-slot, 5-cluster GMM instance on the timed blocks,
+    after the timed region, on one handle: shader-clock cycles per phase summed
+    over waves, exact counts of chain-sum instructions and of their active
+    lanes, and a bit-exact check of the census build's results against the
+    timed run's.  The census build reads the clock at every phase boundary,
+    so its kernel time is reported beside the product build's on the same
+    blocks."""
+    import torch
+    G, S = x.shape
+    total = native.pair_count(G)
+    p = native.make_params(form="persistent", **flags)
+    e = native.Engine(x, p)
+    e.set_tuning(False, 4)
+    ctr = torch.zeros(native.CENSUS_COUNTERS, dtype=torch.int64, device="cuda:0")
+    try:
+        e.set_census(ctr.data_ptr())
+    except native.KincError as err:
+        e.close()
+        return {"skipped": str(err)}
+    e.set_profiling(True)
+    e.profile_read()
+    same = True
+    for i, b in enumerate(blocks):
+        start = b * BLOCK
+        count = min(BLOCK, total - start)
+        K, L, C, I = e.execute(start, count)
+        same = same and np.array_equal(K, out["K"][i][:count]) and \
+            np.array_equal(L, out["labels"][i][:count]) and \
+            np.array_equal(I, out["iters"][i][:count]) and \
+            np.array_equal(C.view(np.uint32), out["corr"][i][:count].view(np.uint32))
+    prof_c = e.profile_read()
+    e.set_census(None)
+    for i, b in enumerate(blocks):
+        start = b * BLOCK
+        e.execute(start, min(BLOCK, total - start))
+    prof_p = e.profile_read()
+    e.set_profiling(False)
+    e.close()
+    c = native.census_dict(ctr.cpu().numpy())
+    cyc = c["cycles"]
+    wave = max(cyc["wave"], 1)
+    named = sum(v for k, v in cyc.items() if k != "wave") + c["queue_cycles"]
+    pairs = max(c["pairs"], 1)
+    res = {
+        "kernel": "k_pairs_census<P=7,KMAX=5> (instrumented build of k_pairs<7,5,4>)",
+        "results_identical_to_timed_run": bool(same),
+        "kernel_ms": {"census": prof_c.get(7, (0, 0.0))[1], "product": prof_p.get(7, (0, 0.0))[1]},
+        "pairs": c["pairs"],
+        "cycle_share": dict({k: v / wave for k, v in cyc.items() if k != "wave"},
+                            queue=c["queue_cycles"] / wave, unattributed=(wave - named) / wave),
+        "chain_sum_share_of_cycles": (cyc["a_chain"] + cyc["b_chain"]) / wave,
+        "chain_lane_utilization": c["chain_lanes"] / max(64 * c["chain_adds"], 1),
+        "per_pair": {k: c[k] / pairs for k in ("fits", "iters", "estep_slots", "a_slots",
+                                               "b_slots", "chain_adds", "chain_calls", "exps",
+                                               "logs")},
+        "counters": c,
+    }
+    return res
+
+
 def cpu_baseline(oracle, native, x, flags, blocks, seconds, gpu_out):
     """Oracle (single thread) on a prefix of every timed block, sized so the
     whole sample takes about `seconds` of CPU time; each prefix starts at its
@@ -614,6 +675,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the handles' default launch configuration (no kinc_sim_autotune)")
     ap.add_argument("--no-rmt", action="store_true", help="skip the RMT threshold-sweep leg")
+    ap.add_argument("--no-census", action="store_true",
+                    help="skip the phase census of the 7-slot GMM kernel")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the config A eager-versus-HIP-graph leg")
     ap.add_argument("--serve-queries", type=int, default=8)
@@ -760,6 +823,11 @@ def main():
     profiled = (args.config == "H" and not args.samples and not args.clusmethod
                 and not args.pearson_fp64 and not args.set)
 
+    census_obj = None
+    if clustered and flags.get("clusmethod") == "gmm" and not args.no_census:
+        census_obj = census_leg(native, x, flags, blocks, out)
+        log("census: %s" % json.dumps({k: v for k, v in census_obj.items() if k != "counters"}))
+
     graph_obj = None
     if not args.no_graph:
         graph_obj = graph_leg(native, gemgen, torch)
@@ -854,6 +922,7 @@ def main():
         "corr_error_vs_fp64": fp64_err,
         "serving": serving_obj,
         "graph": graph_obj,
+        "census": census_obj,
         "rmt": rmt_obj,
     }
     print(json.dumps(line), flush=True)

@@ -71,6 +71,7 @@ SIGNATURES = [
     ("kinc_selftest_math", ctypes.c_int, [ctypes.c_int32, _P, _P, ctypes.c_int64]),
     ("kinc_selftest_valu_probe", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+    ("kinc_selftest_census", ctypes.c_int, [_P, _P]),
     ("kinc_sim_autotune", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P]),
     ("kinc_sim_get_tuning", ctypes.c_int, [_P, _P]),
     ("kinc_sim_set_tuning", ctypes.c_int, [_P, _P]),
@@ -261,6 +262,13 @@ class Engine:
         t.aux_streams = aux_streams
         self._check(lib().kinc_sim_set_tuning(self.h, ctypes.byref(t)))
 
+    def set_census(self, counters_ptr):
+        """Run the 7-slot, 5-cluster pairs on the phase-census build, adding to
+        CENSUS_COUNTERS device uint64 counters at counters_ptr (zeroed by the
+        caller), or on the product build again with None
+        (kinc_selftest_census)."""
+        self._check(lib().kinc_selftest_census(self.h, counters_ptr))
+
     def close(self):
         if getattr(self, "h", None):
             lib().kinc_sim_destroy(self.h)
@@ -305,6 +313,24 @@ def device_math(fn, x):
     if rc != KINC_OK:
         raise KincError(rc, "kinc_selftest_math failed")
     return out
+
+
+# counters of the phase census (include/kinc_selftest.h): cycle phases 0..15,
+# event counts 16..27
+CENSUS_COUNTERS = 32
+CENSUS_PHASES = ["fetch", "fit_init", "estep", "gmax", "a_terms", "a_chain", "mstep",
+                 "b_terms", "b_chain", "cov", "labels", "entropy", "select", "corr",
+                 "other_fit", "wave"]
+CENSUS_EVENTS = ["pairs", "fits", "iters", "estep_slots", "a_slots", "b_slots", "chain_adds",
+                 "chain_lanes", "chain_calls", "exps", "logs", "queue_cycles"]
+
+
+def census_dict(values):
+    """Name the KINC_CENSUS_COUNTERS values of a census run."""
+    v = [int(x) for x in values]
+    d = {"cycles": {n: v[i] for i, n in enumerate(CENSUS_PHASES)}}
+    d.update({n: v[16 + i] for i, n in enumerate(CENSUS_EVENTS)})
+    return d
 
 
 # (independent chains, active lanes, instruction) of each kinc_selftest_valu_probe mode

@@ -54,6 +54,9 @@ INSTANCES = ([(5, p) for p in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16)]
 # 500 samples, where LDS allows 16, mixing them into the concurrent bins
 # cost time
 INSTANCES_W5 = [(5, p) for p in (1, 2, 3, 4, 5, 6)] + [(8, p) for p in (2, 4)]
+# the phase-census build (per-phase cycle and event counters, profiling only)
+# of the instance that carries the 20,000 x 500 GMM workload
+CENSUS_INSTANCES = [(5, 7)]
 CLI_SOURCE = "kinc_cli.cpp"
 CLI = os.path.join(HERE, "kinc")
 
@@ -76,6 +79,9 @@ def build(verbose=False):
     jobs += [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p,
                                      "KINC_PAIR_WAVES=5"),
               "pairs_k%d_p%d_w5" % (k, p)) for k, p in INSTANCES_W5]
+    jobs += [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p,
+                                     "KINC_PAIR_WAVES=%d" % PAIR_WAVES, "KINC_CENSUS=1"),
+              "pairs_census_k%d_p%d" % (k, p)) for k, p in CENSUS_INSTANCES]
     jobs += [(src, ("KINC_PAIR_WAVES=%d" % PAIR_WAVES,), None) for src in SOURCES]
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with ThreadPoolExecutor(max_workers=workers) as ex:

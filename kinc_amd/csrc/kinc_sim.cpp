@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/kinc_selftest.h"
 #include "../../include/kinc_sim.h"
 #include "sim_device.h"
 
@@ -43,6 +44,7 @@ const void* kinc_kernel_prev_greater();
 const void* kinc_kernel_pearson_lanes();
 const void* kinc_kernel_bin_pairs();
 const void* kinc_kernel_zero_i32();
+const void* kinc_pair_kernel_census_k5_p7();
 }
 
 namespace {
@@ -165,6 +167,14 @@ struct kinc_sim
    std::vector<Timed> timed;
    std::vector<hipEvent_t> event_pool;
 };
+
+/* The pair kernel a launch of slot count P uses: the phase-census build
+ * while a census is on and one exists for P, else the product build. */
+static const void* pair_kernel(const kinc_sim* h, int P)
+{
+   if ( h->base.census && P == 7 && h->kmax == 5 && !h->w5 ) return kinc_pair_kernel_census_k5_p7();
+   return kinc_select_pair_kernel(P, h->kmax, h->w5);
+}
 
 static int fail(kinc_sim* h, int code, const std::string& msg)
 {
@@ -603,7 +613,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
    {
       /* the slot count S needs (or the memory store) for every pair */
       const int P = h->classes[h->nbins - 1];
-      const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
+      const void* fn = pair_kernel(h, P);
       if ( !fn ) return fail_c(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       a.list = nullptr;
       for ( int first = 0; first < cnt; first += h->p.ksize )
@@ -633,7 +643,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       /* no fits: the slot count does not matter and every pair costs about
        * the same, so resident waves stride over the whole block in one launch */
       const int P = h->classes[0];
-      const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
+      const void* fn = pair_kernel(h, P);
       if ( !fn ) return fail_c(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
       HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
@@ -680,7 +690,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
    {
       const int P = h->classes[b];
       hipStream_t sb = fork ? h->aux[(h->nbins - 1 - b) % naux] : s;
-      const void* fn = kinc_select_pair_kernel(P, h->kmax, h->w5);
+      const void* fn = pair_kernel(h, P);
       if ( !fn ) return fail_c(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
       HIP_TRY(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, h->lds_bytes));
@@ -949,6 +959,20 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
       out->candidates = tried;
       out->sample_ms = best_ms;
    }
+   return KINC_OK;
+}
+
+extern "C" int kinc_selftest_census(kinc_sim* h, uint64_t* d_counters)
+{
+   if ( !h ) return KINC_E_INVALID;
+   if ( h->capturing ) return fail_c(h, KINC_E_INVALID, "the census cannot change while a graph is captured");
+   if ( d_counters )
+   {
+      bool has7 = false;
+      for ( int b = 0; b < h->nbins; ++b ) has7 = has7 || h->classes[b] == 7;
+      if ( h->kmax != 5 || !has7 ) return fail_c(h, KINC_E_UNSUPPORTED, "the census build covers the 7-slot, 5-cluster instance only");
+   }
+   h->base.census = (unsigned long long*)d_counters;
    return KINC_OK;
 }
 

@@ -100,6 +100,7 @@ struct SimArgs
    int32_t mem_slots;        /* ceil(S / 64) */
    int32_t np2;              /* next power of two >= S (min 2) */
    int32_t tile_stride;      /* KINC_TILE_STRIDE */
+   unsigned long long* census;  /* phase-census build only: its counters */
 };
 
 #endif

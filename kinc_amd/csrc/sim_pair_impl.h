@@ -35,6 +35,9 @@
 
 #include "kinc_math.h"
 #include "sim_device.h"
+#ifdef KINC_CENSUS
+#include "../../include/kinc_selftest.h"
+#endif
 
 #define WAVE 64
 #define TILE 64
@@ -149,6 +152,38 @@ __device__ __forceinline__ void wave_sync(bool global_mem)
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
    }
 }
+
+/*
+ * Phase census (instrumented build only, KINC_CENSUS defined): per-wave
+ * counters in static LDS, added to by lane 0 with LDS atomics (no return, so
+ * no wait), flushed to global memory when the wave exits.  A lap reads the
+ * shader clock (s_memtime, which waits for the wave's outstanding LDS and
+ * scalar loads) and charges the cycles since the previous lap to a phase.
+ * Without KINC_CENSUS every macro expands to nothing.
+ */
+#ifdef KINC_CENSUS
+__shared__ unsigned long long g_census[KINC_CENSUS_COUNTERS];
+
+__device__ __forceinline__ void census_add(int c, uint64_t v)
+{
+   if ( lane_id() == 0 ) atomicAdd(&g_census[c], (unsigned long long)v);
+}
+#define CENSUS_T(t) uint64_t t = __builtin_amdgcn_s_memtime()
+#define CENSUS_RESET(t) (t = __builtin_amdgcn_s_memtime())
+#define CENSUS_LAP(ph, t)                                                     \
+   do                                                                         \
+   {                                                                          \
+      const uint64_t n_ = __builtin_amdgcn_s_memtime();                       \
+      census_add((ph), n_ - (t));                                             \
+      (t) = n_;                                                               \
+   } while ( 0 )
+#define CENSUS_COUNT(c, v) census_add((c), (uint64_t)(v))
+#else
+#define CENSUS_T(t) ((void)0)
+#define CENSUS_RESET(t) ((void)0)
+#define CENSUS_LAP(ph, t) ((void)0)
+#define CENSUS_COUNT(c, v) ((void)0)
+#endif
 
 __device__ __forceinline__ int next_pow2(int n)
 {
@@ -435,6 +470,9 @@ __device__ __forceinline__ float chain_add(const float* tile, int stride, int nc
    const int lane = lane_id();
    if ( lane < nch || lane < KINC_MIN_ACTIVE_LANES )
    {
+      CENSUS_COUNT(KINC_CENSUS_CHAIN_ADDS, len);
+      CENSUS_COUNT(KINC_CENSUS_CHAIN_LANES, (uint64_t)len * (uint64_t)__builtin_popcountll(__builtin_amdgcn_read_exec()));
+      CENSUS_COUNT(KINC_CENSUS_CHAIN_CALLS, 1);
       const float* row = tile + (lane < nch ? lane : nch - 1) * stride;
       if ( len == TILE )
       {
@@ -737,6 +775,8 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
    out.ok = true;
    out.logL = 0.0f;
    out.E = 0.0f;
+   CENSUS_T(ct);
+   CENSUS_COUNT(KINC_CENSUS_FITS, 1);
 
    /* initialisation (pairwise_gmm.cpp:372-392): K draws of rand() % N */
    {
@@ -759,6 +799,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       }
       wave_sync(L.lgbuf != nullptr);
    }
+   CENSUS_LAP(KINC_CENSUS_FIT_INIT, ct);
 
    float prevLogL = -INFINITY;
    float curLogL = -INFINITY;
@@ -771,6 +812,11 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       float vmax[K];
       gmm_estep<ST, K>(nslots, st, prm, vmax, ST::PMAX != 0);
       e_steps++;
+      CENSUS_LAP(KINC_CENSUS_ESTEP, ct);
+      CENSUS_COUNT(KINC_CENSUS_ITERS, 1);
+      CENSUS_COUNT(KINC_CENSUS_ESTEP_SLOTS, nslots);
+      CENSUS_COUNT(KINC_CENSUS_EXPS, K * nslots);
+      CENSUS_COUNT(KINC_CENSUS_LOGS, nslots);
 
       /* max over the points of each component's log responsibility; only
        * the last slot holds lanes past N.  fmaxf as in the E step: v is never
@@ -801,6 +847,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          }
          gmax[k] = wave_max(v);
       }
+      CENSUS_LAP(KINC_CENSUS_GMAX, ct);
 
       /* pass A: chain 0 = logL, 1+k = Gamma_k sum, 1+K+2k / 2+K+2k = mu_k sums.
        * The memory store recomputes the log responsibilities from X and the
@@ -847,8 +894,12 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
             /* opaque to loop-invariant hoisting: the partial-tile values of
              * every unrolled slot would otherwise live across the EM loop */
             asm volatile("" : "+s"(len));
+            CENSUS_LAP(KINC_CENSUS_A_TERMS, ct);
             acc = chain_add(tile, STR, 1 + 3 * K, len, acc);
             wave_sync(L.lgbuf != nullptr);
+            CENSUS_LAP(KINC_CENSUS_A_CHAIN, ct);
+            CENSUS_COUNT(KINC_CENSUS_A_SLOTS, 1);
+            CENSUS_COUNT(KINC_CENSUS_EXPS, 2 * K);
          }
       }
 
@@ -883,6 +934,8 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
 #pragma unroll
       for ( int k = 0; k < K; ++k ) sum = sum + bcast(ek, k);
       const float logGammaSum = mx + kinc_logf_ge1(sum);
+      CENSUS_COUNT(KINC_CENSUS_EXPS, 2);
+      CENSUS_COUNT(KINC_CENSUS_LOGS, 2);
 
       const float Gam = kinc_expf(logGamma);
       const float cm = 1.0f / Gam;
@@ -897,6 +950,8 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          p[PRM_LPI] = lpk + (logGamma - logGammaSum);
       }
       wave_sync(L.lgbuf != nullptr);
+
+      CENSUS_LAP(KINC_CENSUS_MSTEP, ct);
 
       /* pass B: chain 3k / 3k+1 / 3k+2 = Sigma_k entries 00, 01 (= 10), 11 */
       float accB = 0.0f;
@@ -936,8 +991,11 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
             /* opaque to loop-invariant hoisting: the partial-tile values of
              * every unrolled slot would otherwise live across the EM loop */
             asm volatile("" : "+s"(len));
+            CENSUS_LAP(KINC_CENSUS_B_TERMS, ct);
             accB = chain_add(tile, STR, 3 * K, len, accB);
             wave_sync(L.lgbuf != nullptr);
+            CENSUS_LAP(KINC_CENSUS_B_CHAIN, ct);
+            CENSUS_COUNT(KINC_CENSUS_B_SLOTS, 1);
          }
       }
 
@@ -957,6 +1015,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          p[PRM_NRM] = nrm;
       }
       wave_sync(L.lgbuf != nullptr);
+      CENSUS_LAP(KINC_CENSUS_COV, ct);
    }
 
    out.iters = (uint16_t)((e_steps & 0x0fff) | (converged ? (1u << 12) : 0u) | (failed ? (1u << 13) : 0u));
@@ -974,7 +1033,11 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
    if ( converged )
    {
       gmm_estep<ST, K>(nslots, st, prm);
+      CENSUS_COUNT(KINC_CENSUS_ESTEP_SLOTS, nslots);
+      CENSUS_COUNT(KINC_CENSUS_EXPS, K * nslots);
+      CENSUS_COUNT(KINC_CENSUS_LOGS, nslots);
    }
+   if ( want_entropy ) CENSUS_COUNT(KINC_CENSUS_LOGS, nslots);
 
    /* calcLabels (pairwise_gmm.cpp:323-341); the entropy terms (:348-360)
     * go to LPX, which is free after the last E step */
@@ -1001,6 +1064,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       }
    }
    wave_sync(L.lgbuf != nullptr);
+   CENSUS_LAP(KINC_CENSUS_LABELS, ct);
 
    if ( want_entropy )
    {
@@ -1022,6 +1086,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
          }
       }
       out.E = bcast(accE, 0);
+      CENSUS_LAP(KINC_CENSUS_ENTROPY, ct);
    }
    return out;
 }
@@ -1652,6 +1717,7 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
    const int S = a.S;
    const int MC = a.max_k;
 
+   CENSUS_T(cp);
    int gx, gy;
    pair_genes(a, q, gx, gy);
    const int N = fetch_pair(a, gx, gy, L);
@@ -1668,6 +1734,8 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
          mark_outliers(L, N, 1, 0, -7);
       }
       int bestK = 0;
+      CENSUS_LAP(KINC_CENSUS_FETCH, cp);
+      CENSUS_COUNT(KINC_CENSUS_PAIRS, 1);
       if ( N >= a.min_samples )
       {
          float bestValue = INFINITY;
@@ -1683,6 +1751,8 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
             {
                f = fit_dispatch<P, KMAX>(a, L, N, Kc);
             }
+            if ( a.clus == 1 ) CENSUS_RESET(cp);
+            else CENSUS_LAP(KINC_CENSUS_OTHER_FIT, cp);
             if ( lane == Kc - 1 ) itv = f.iters;
             if ( !f.ok ) continue;
 
@@ -1708,6 +1778,7 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
          }
       }
       K = bestK;
+      CENSUS_LAP(KINC_CENSUS_SELECT, cp);
    }
 
    if constexpr ( KMAX > 8 )
@@ -1783,6 +1854,7 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
       for ( int s = lane; s < S; s += WAVE ) lo[s] = L.lab[s];
    }
    wave_sync(L.lgbuf != nullptr);
+   CENSUS_LAP(KINC_CENSUS_CORR, cp);
    }
 }
 
@@ -1798,12 +1870,22 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
  * pair ids from a device queue until it is empty, so every wave reaches the
  * exit test.
  */
+#ifdef KINC_CENSUS
+#define KINC_PAIR_KERNEL k_pairs_census
+#else
+#define KINC_PAIR_KERNEL k_pairs
+#endif
 template <int P, int KMAX, int W>
-__global__ __launch_bounds__(64, W) void k_pairs(SimArgs a)
+__global__ __launch_bounds__(64, W) void KINC_PAIR_KERNEL(SimArgs a)
 {
    /* one loop for both: the next pair is the workgroup's next stride (no
     * list) or the next id popped from the bin's queue, so the per-pair code
     * is inlined once */
+#ifdef KINC_CENSUS
+   if ( lane_id() < KINC_CENSUS_COUNTERS ) g_census[lane_id()] = 0;
+   wave_sync(false);
+#endif
+   CENSUS_T(kt);
    const bool queued = a.list != nullptr;
    const int len = queued ? *a.list_len : 0;
    const int end = a.first + a.launch_count;
@@ -1819,14 +1901,21 @@ __global__ __launch_bounds__(64, W) void k_pairs(SimArgs a)
       }
       else
       {
+         CENSUS_T(tq);
          int idx = 0;
          if ( lane_id() == 0 ) idx = atomicAdd(a.list_head, 1);
          idx = bcast_i(idx, 0);
+         CENSUS_LAP(KINC_CENSUS_QUEUE, tq);
          if ( idx >= len ) break;
          pair = a.list[idx];
       }
       process_pair<P, KMAX>(a, pair);
    }
+#ifdef KINC_CENSUS
+   CENSUS_LAP(KINC_CENSUS_WAVE, kt);
+   wave_sync(false);
+   if ( lane_id() < KINC_CENSUS_COUNTERS ) atomicAdd(&a.census[lane_id()], g_census[lane_id()]);
+#endif
 }
 
 #endif

@@ -4,7 +4,8 @@
  * Compiled once per (KINC_INST_P, KINC_INST_KMAX, KINC_PAIR_WAVES) by
  * kinc_amd/build.py so that the instances build in parallel; each exports a
  * getter named kinc_pair_kernel_k<KMAX>_p<P> (4 waves per SIMD) or
- * kinc_pair_kernel_k<KMAX>_p<P>_w5 (5) returning the kernel's host handle.
+ * kinc_pair_kernel_k<KMAX>_p<P>_w5 (5) returning the kernel's host handle;
+ * with KINC_CENSUS, the instrumented build kinc_pair_kernel_census_k<KMAX>_p<P>.
  */
 #include "sim_pair_impl.h"
 
@@ -18,7 +19,13 @@
 #define KINC_CAT2(a, b, c, d) a##b##c##d
 #define KINC_CAT(a, b, c, d) KINC_CAT2(a, b, c, d)
 
-#if KINC_PAIR_WAVES == 4
+#if defined(KINC_CENSUS)
+/* the phase-census build of an instance (profiling only) */
+extern "C" const void* KINC_CAT(kinc_pair_kernel_census_k, KINC_INST_KMAX, _p, KINC_INST_P)()
+{
+   return (const void*)&k_pairs_census<KINC_INST_P, KINC_INST_KMAX, KINC_PAIR_WAVES>;
+}
+#elif KINC_PAIR_WAVES == 4
 extern "C" const void* KINC_CAT(kinc_pair_kernel_k, KINC_INST_KMAX, _p, KINC_INST_P)()
 #else
 /* the 5-waves-per-SIMD build of a small instance */
@@ -26,6 +33,8 @@ extern "C" const void* KINC_CAT(kinc_pair_kernel_k, KINC_INST_KMAX, _p, KINC_INS
 #define KINC_CATW(a, b, c, d, e) KINC_CAT3(a, b, c, d, e)
 extern "C" const void* KINC_CATW(kinc_pair_kernel_k, KINC_INST_KMAX, _p, KINC_INST_P, _w5)()
 #endif
+#if !defined(KINC_CENSUS)
 {
    return (const void*)&k_pairs<KINC_INST_P, KINC_INST_KMAX, KINC_PAIR_WAVES>;
 }
+#endif
