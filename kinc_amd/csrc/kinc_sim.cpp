@@ -331,13 +331,17 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
    off = align16(off + (size_t)S);
    if ( h->classes[0] != 0 )
    {
-      /* everything in LDS; the chain tile shares the sort-buffer region */
+      /* everything in LDS; the chain tile shares the sort-buffer region.  The
+       * GMM fits with 2 and 3 clusters run together on 17 chain rows, and the
+       * second fit's labels wait past the tile's first row. */
       a.scratch = nullptr;
       a.lds_prm = (int32_t)off;
       off = align16(off + sizeof(float) * 8 * 8);
       a.lds_u = (int32_t)off;
       a.lds_tile = a.lds_u;
-      off += std::max(tile_bytes, sort_bytes);
+      const size_t joint_bytes = std::max(sizeof(float) * 17 * (size_t)a.tile_stride,
+                                          sizeof(float) * (size_t)a.tile_stride + (size_t)S);
+      off += std::max(std::max(tile_bytes, joint_bytes), sort_bytes);
       h->lds_bytes = align16(off);
       h->w5_fits = h->lds_bytes * 20 <= (size_t)160 * 1024;
       h->w5 = h->w5_fits;

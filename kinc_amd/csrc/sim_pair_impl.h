@@ -408,7 +408,7 @@ __device__ __forceinline__ void mark_outliers(const Lds& L, int N, int j, int8_t
 
 /* pairwise_clustering.cpp:77-84: labels[i] >= 0 (i < N) take the fit's
  * labels in compact order. */
-__device__ __forceinline__ void copy_back(const Lds& L, int N)
+__device__ __forceinline__ void copy_back(const Lds& L, int N, const int8_t* w)
 {
    const int lane = lane_id();
    int jb = 0;
@@ -418,7 +418,7 @@ __device__ __forceinline__ void copy_back(const Lds& L, int N)
       int8_t l = (i < N) ? L.lab[i] : (int8_t)-1;
       bool ge = l >= 0;
       uint64_t m = ballot(ge);
-      if ( ge ) L.lab[i] = L.w[jb + popc(m & lanes_below())];
+      if ( ge ) L.lab[i] = w[jb + popc(m & lanes_below())];
       jb += popc(m);
    }
    wave_sync(L.lgbuf != nullptr);
@@ -1116,6 +1116,434 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
    return out;
 }
 
+/* ------------------------------------------------------ GMM, two fits */
+
+/*
+ * Two GMM fits of one pair (KA and KB components, KA + KB <= 5) run in
+ * lockstep: the same EM arithmetic per fit as gmm_fit, in the same order per
+ * value, but one chain tile holds both fits' rows, so each pass's sequential
+ * chain sums (64 dependent adds per slot, the latency-bound part of an
+ * iteration) and the per-component M-step and covariance work serve both
+ * fits at once.  A fit that converges or fails stops (its E step and terms
+ * are skipped, its parameters frozen) while the other runs on.  Component k
+ * of the first fit is parameter set k; component k of the second is set
+ * KA + k.  Labels of the first fit go to L.w, of the second to wb.
+ */
+template <int P, int KT>
+struct JointStore
+{
+   static constexpr int PMAX = P;
+   float x0[P], x1[P];
+   float lg[KT][P];
+   float lpx[2][P];
+
+   __device__ __forceinline__ void init(const Lds& L, int N)
+   {
+      const int lane = lane_id();
+#pragma unroll
+      for ( int m = 0; m < P; ++m )
+      {
+         const int i = lane + m * WAVE;
+         const float2 v = (i < N) ? L.X[i] : make_float2(0.0f, 0.0f);
+         x0[m] = v.x;
+         x1[m] = v.y;
+      }
+   }
+};
+
+/* one fit's view of a JointStore: components OFF.., log p(x) set F */
+template <int P, int KT, int OFF, int F>
+struct JointView
+{
+   static constexpr int PMAX = P;
+   JointStore<P, KT>& js;
+   __device__ __forceinline__ float X0(int m) const { return js.x0[m]; }
+   __device__ __forceinline__ float X1(int m) const { return js.x1[m]; }
+   __device__ __forceinline__ float& LG(int k, int m) { return js.lg[OFF + k][m]; }
+   __device__ __forceinline__ float& LPX(int m) { return js.lpx[F][m]; }
+};
+
+/* calcLabels and the entropy terms of one finished fit (as in gmm_fit) */
+template <int P, int KT, int OFF, int F, int K>
+__device__ __forceinline__ float joint_labels(const Lds& L, int N, JointView<P, KT, OFF, F> v,
+                                              const float* prm, bool converged, bool want_entropy,
+                                              int8_t* w)
+{
+   const int lane = lane_id();
+   constexpr int STR = KINC_TILE_STRIDE;
+   const int nslots = (N + WAVE - 1) / WAVE;
+   if ( converged )
+   {
+      gmm_estep<JointView<P, KT, OFF, F>, K>(nslots, v, prm + 8 * OFF);
+      CENSUS_COUNT(KINC_CENSUS_ESTEP_SLOTS, nslots);
+      CENSUS_COUNT(KINC_CENSUS_EXPS, K * nslots);
+      CENSUS_COUNT(KINC_CENSUS_LOGS, nslots);
+   }
+   if ( want_entropy ) CENSUS_COUNT(KINC_CENSUS_LOGS, nslots);
+#pragma unroll
+   for ( int m = 0; m < P; ++m )
+   {
+      const int i = lane + m * WAVE;
+      if ( m < nslots )
+      {
+         int max_k = -1;
+         float mg = -INFINITY;
+#pragma unroll
+         for ( int k = 0; k < K; ++k )
+         {
+            const float l = v.LG(k, m);
+            if ( mg < l )
+            {
+               max_k = k;
+               mg = l;
+            }
+         }
+         if ( i < N ) w[i] = (int8_t)max_k;
+         if ( want_entropy ) v.LPX(m) = (max_k >= 0) ? kinc_logf(mg) : NAN;
+      }
+   }
+   wave_sync(false);
+   float accE = 0.0f;
+   if ( want_entropy )
+   {
+      float* tile = L.tile;
+#pragma unroll
+      for ( int m = 0; m < P; ++m )
+      {
+         if ( m < nslots )
+         {
+            tile[lane] = v.LPX(m);
+            wave_sync(false);
+            int len = N - m * WAVE;
+            len = len < TILE ? len : TILE;
+            asm volatile("" : "+s"(len));
+            accE = chain_add(tile, STR, 1, len, accE);
+            wave_sync(false);
+         }
+      }
+   }
+   return bcast(accE, 0);
+}
+
+/* pass-A terms of one fit into tile rows R.. (lp, K gamma-sum terms, 2K
+ * mean terms) and the exponentiated responsibilities into LG */
+template <class V, int K>
+__device__ __forceinline__ void joint_terms_a(V& v, int m, float* tile, int R, const float* gmax)
+{
+   constexpr int STR = KINC_TILE_STRIDE;
+   const int lane = lane_id();
+   tile[R * STR + lane] = v.LPX(m);
+   const float xa = v.X0(m);
+   const float xb = v.X1(m);
+#pragma unroll
+   for ( int k = 0; k < K; ++k )
+   {
+      const float l = v.LG(k, m);
+      tile[(R + 1 + k) * STR + lane] = kinc_expf_le0(l - gmax[k]);
+      const float gm = kinc_expf(l);
+      v.LG(k, m) = gm;
+      tile[(R + 1 + K + 2 * k) * STR + lane] = gm * xa;
+      tile[(R + 2 + K + 2 * k) * STR + lane] = gm * xb;
+   }
+}
+
+/* pass-B terms of one fit (components OFF..OFF+K-1 of the joint tile) */
+template <class V, int K, int OFF>
+__device__ __forceinline__ void joint_terms_b(V& v, int m, float* tile, const float* muk0, const float* muk1)
+{
+   constexpr int STR = KINC_TILE_STRIDE;
+   const int lane = lane_id();
+   const float xa = v.X0(m);
+   const float xb = v.X1(m);
+#pragma unroll
+   for ( int k = 0; k < K; ++k )
+   {
+      const float xm0 = xa - muk0[OFF + k];
+      const float xm1 = xb - muk1[OFF + k];
+      const float gm = v.LG(k, m);
+      tile[(3 * (OFF + k)) * STR + lane] = gm * (xm0 * xm0);
+      tile[(3 * (OFF + k) + 1) * STR + lane] = gm * (xm0 * xm1);
+      tile[(3 * (OFF + k) + 2) * STR + lane] = gm * (xm1 * xm1);
+   }
+}
+
+/* per-component maximum of one fit's log responsibilities (as gmm_fit) */
+template <class V, int K>
+__device__ __forceinline__ void joint_gmax(V& v, int N, int nslots, float* gmax)
+{
+   const int lane = lane_id();
+#pragma unroll
+   for ( int k = 0; k < K; ++k )
+   {
+      float x = -INFINITY;
+#pragma unroll
+      for ( int m = 0; m < V::PMAX; ++m )
+      {
+         if ( m < nslots - 1 )
+         {
+            x = fmaxf(x, v.LG(k, m));
+         }
+         else if ( m == nslots - 1 )
+         {
+            const float l = v.LG(k, m);
+            x = (lane + m * WAVE < N) ? fmaxf(x, l) : x;
+         }
+      }
+      gmax[k] = wave_max(x);
+   }
+}
+
+template <int P, int KA, int KB>
+__device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_entropy, FitOut& fa,
+                                              FitOut& fb, int8_t* wb)
+{
+   constexpr int KT = KA + KB;
+   static_assert(KT <= 5, "two fits share at most five components");
+   constexpr int STR = KINC_TILE_STRIDE;
+   constexpr int RB = 1 + 3 * KA;          /* first pass-A row of the second fit */
+   constexpr int NCA = RB + 1 + 3 * KB;    /* pass-A chains */
+   const int lane = lane_id();
+   const int nslots = (N + WAVE - 1) / WAVE;
+   float* tile = L.tile;
+   float* prm = L.prm;
+   CENSUS_T(ct);
+   CENSUS_COUNT(KINC_CENSUS_FITS, 2);
+
+   JointStore<P, KT> js;
+   js.init(L, N);
+   JointView<P, KT, 0, 0> va{js};
+   JointView<P, KT, KA, 1> vb{js};
+
+   /* initialisation (pairwise_gmm.cpp:372-392), each fit with its own LCG */
+   {
+      uint64_t ra = 1, rb = 1;
+      int my = 0;
+#pragma unroll
+      for ( int k = 0; k < KA; ++k )
+      {
+         const int i = lcg(ra) % N;
+         my = (lane == k) ? i : my;
+      }
+#pragma unroll
+      for ( int k = 0; k < KB; ++k )
+      {
+         const int i = lcg(rb) % N;
+         my = (lane == KA + k) ? i : my;
+      }
+      if ( lane < KT )
+      {
+         float2 c = L.X[my];
+         float a00, a01, a10, a11, nrm;
+         prepare_cov(1.0f, 0.0f, 0.0f, 1.0f, a00, a01, a10, a11, nrm);
+         float* p = prm + 8 * lane;
+         *(float4*)p = make_float4(c.x, c.y, a00, a01);
+         *(float4*)(p + 4) = make_float4(a10, a11, nrm, kinc_logf(1.0f / (lane < KA ? KA : KB)));
+      }
+      wave_sync(false);
+   }
+   CENSUS_LAP(KINC_CENSUS_FIT_INIT, ct);
+
+   float prevA = -INFINITY, curA = -INFINITY, prevB = -INFINITY, curB = -INFINITY;
+   int eA = 0, eB = 0;
+   bool actA = true, actB = true;
+   bool convA = false, convB = false, failA = false, failB = false;
+
+   for ( int t = 0; t < 100; ++t )
+   {
+      if ( actA )
+      {
+         gmm_estep<JointView<P, KT, 0, 0>, KA>(nslots, va, prm);
+         eA++;
+         CENSUS_COUNT(KINC_CENSUS_ESTEP_SLOTS, nslots);
+         CENSUS_COUNT(KINC_CENSUS_EXPS, KA * nslots);
+         CENSUS_COUNT(KINC_CENSUS_LOGS, nslots);
+      }
+      if ( actB )
+      {
+         gmm_estep<JointView<P, KT, KA, 1>, KB>(nslots, vb, prm + 8 * KA);
+         eB++;
+         CENSUS_COUNT(KINC_CENSUS_ESTEP_SLOTS, nslots);
+         CENSUS_COUNT(KINC_CENSUS_EXPS, KB * nslots);
+         CENSUS_COUNT(KINC_CENSUS_LOGS, nslots);
+      }
+      CENSUS_LAP(KINC_CENSUS_ESTEP, ct);
+      CENSUS_COUNT(KINC_CENSUS_ITERS, (actA ? 1 : 0) + (actB ? 1 : 0));
+
+      float gmax[KT];
+#pragma unroll
+      for ( int k = 0; k < KT; ++k ) gmax[k] = 0.0f;
+      if ( actA ) joint_gmax<JointView<P, KT, 0, 0>, KA>(va, N, nslots, gmax);
+      if ( actB ) joint_gmax<JointView<P, KT, KA, 1>, KB>(vb, N, nslots, gmax + KA);
+      CENSUS_LAP(KINC_CENSUS_GMAX, ct);
+
+      /* pass A: rows 0..RB-1 the first fit, RB.. the second (rows of a fit
+       * that has stopped hold stale values whose sums are not read) */
+      float acc = 0.0f;
+#pragma unroll
+      for ( int m = 0; m < P; ++m )
+      {
+         if ( m < nslots )
+         {
+            if ( actA ) joint_terms_a<JointView<P, KT, 0, 0>, KA>(va, m, tile, 0, gmax);
+            if ( actB ) joint_terms_a<JointView<P, KT, KA, 1>, KB>(vb, m, tile, RB, gmax + KA);
+            CENSUS_COUNT(KINC_CENSUS_A_SLOTS, (actA ? 1 : 0) + (actB ? 1 : 0));
+            CENSUS_COUNT(KINC_CENSUS_EXPS, (actA ? 2 * KA : 0) + (actB ? 2 * KB : 0));
+            wave_sync(false);
+            int len = N - m * WAVE;
+            len = len < TILE ? len : TILE;
+            /* opaque to loop-invariant hoisting, as in gmm_fit */
+            asm volatile("" : "+s"(len));
+            CENSUS_LAP(KINC_CENSUS_A_TERMS, ct);
+            acc = chain_add(tile, STR, NCA, len, acc);
+            wave_sync(false);
+            CENSUS_LAP(KINC_CENSUS_A_CHAIN, ct);
+         }
+      }
+
+      if ( actA )
+      {
+         prevA = curA;
+         curA = bcast(acc, 0);
+         if ( fabsf(curA - prevA) < 1e-8f )
+         {
+            convA = true;
+            actA = false;
+         }
+      }
+      if ( actB )
+      {
+         prevB = curB;
+         curB = bcast(acc, RB);
+         if ( fabsf(curB - prevB) < 1e-8f )
+         {
+            convB = true;
+            actB = false;
+         }
+      }
+      if ( !actA && !actB ) break;
+
+      /* M step (pairwise_gmm.cpp:194-316): component k of the first fit on
+       * lane k, of the second on lane KA + k; each fit's log-sum-exp over its
+       * own lanes, in component order */
+      const bool inA = lane < KA;
+      const int rg = inA ? 1 + lane : RB + 1 + (lane - KA);
+      const int rm = inA ? 1 + KA + 2 * lane : RB + 1 + KB + 2 * (lane - KA);
+      const float gsum = __shfl(acc, rg);
+      const float mxs = __shfl(acc, rm);
+      const float mys = __shfl(acc, rm + 1);
+      const float logGamma = lane_pick<KT>(gmax) + kinc_logf_ge1(gsum);
+      const float lpk = prm[8 * (lane < KT ? lane : 0) + PRM_LPI];
+      const float v = lpk + logGamma;
+      float mxA = -INFINITY, mxB = -INFINITY;
+#pragma unroll
+      for ( int k = 0; k < KA; ++k )
+      {
+         const float vk = bcast(v, k);
+         mxA = (vk > mxA) ? vk : mxA;
+      }
+#pragma unroll
+      for ( int k = 0; k < KB; ++k )
+      {
+         const float vk = bcast(v, KA + k);
+         mxB = (vk > mxB) ? vk : mxB;
+      }
+      const float mx = inA ? mxA : mxB;
+      const float ek = kinc_expf_le0(v - mx);
+      float sumA = 0.0f, sumB = 0.0f;
+#pragma unroll
+      for ( int k = 0; k < KA; ++k ) sumA = sumA + bcast(ek, k);
+#pragma unroll
+      for ( int k = 0; k < KB; ++k ) sumB = sumB + bcast(ek, KA + k);
+      const float logGammaSum = mx + kinc_logf_ge1(inA ? sumA : sumB);
+      CENSUS_COUNT(KINC_CENSUS_EXPS, 2);
+      CENSUS_COUNT(KINC_CENSUS_LOGS, 2);
+
+      const float Gam = kinc_expf(logGamma);
+      const float cm = 1.0f / Gam;
+      const float mu0 = mxs * cm;
+      const float mu1 = mys * cm;
+      const bool mine = lane < KT && (inA ? actA : actB);
+      wave_sync(false);
+      if ( mine )
+      {
+         float* p = prm + 8 * lane;
+         p[PRM_MU0] = mu0;
+         p[PRM_MU1] = mu1;
+         p[PRM_LPI] = lpk + (logGamma - logGammaSum);
+      }
+      wave_sync(false);
+      CENSUS_LAP(KINC_CENSUS_MSTEP, ct);
+
+      /* pass B: rows 3c, 3c+1, 3c+2 = Sigma entries of parameter set c */
+      float accB = 0.0f;
+      float muk0[KT], muk1[KT];
+#pragma unroll
+      for ( int k = 0; k < KT; ++k )
+      {
+         const float2 mu = *(const float2*)(prm + 8 * k);
+         muk0[k] = mu.x;
+         muk1[k] = mu.y;
+      }
+#pragma unroll
+      for ( int m = 0; m < P; ++m )
+      {
+         if ( m < nslots )
+         {
+            if ( actA ) joint_terms_b<JointView<P, KT, 0, 0>, KA, 0>(va, m, tile, muk0, muk1);
+            if ( actB ) joint_terms_b<JointView<P, KT, KA, 1>, KB, KA>(vb, m, tile, muk0, muk1);
+            CENSUS_COUNT(KINC_CENSUS_B_SLOTS, (actA ? 1 : 0) + (actB ? 1 : 0));
+            wave_sync(false);
+            int len = N - m * WAVE;
+            len = len < TILE ? len : TILE;
+            asm volatile("" : "+s"(len));
+            CENSUS_LAP(KINC_CENSUS_B_TERMS, ct);
+            accB = chain_add(tile, STR, 3 * KT, len, accB);
+            wave_sync(false);
+            CENSUS_LAP(KINC_CENSUS_B_CHAIN, ct);
+         }
+      }
+
+      const float s00 = __shfl(accB, 3 * lane) * cm;
+      const float s01 = __shfl(accB, 3 * lane + 1) * cm;
+      const float s11 = __shfl(accB, 3 * lane + 2) * cm;
+      float a00, a01, a10, a11, nrm = 0.0f;
+      const bool ok = prepare_cov(s00, s01, s01, s11, a00, a01, a10, a11, nrm);
+      if ( actA && ballot(inA && !ok) != 0 )
+      {
+         failA = true;
+         actA = false;
+      }
+      if ( actB && ballot(!inA && lane < KT && !ok) != 0 )
+      {
+         failB = true;
+         actB = false;
+      }
+      if ( lane < KT && (inA ? actA : actB) )
+      {
+         float* p = prm + 8 * lane;
+         *(float4*)(p + 0) = make_float4(mu0, mu1, a00, a01);
+         p[PRM_A10] = a10;
+         p[PRM_A11] = a11;
+         p[PRM_NRM] = nrm;
+      }
+      wave_sync(false);
+      CENSUS_LAP(KINC_CENSUS_COV, ct);
+      if ( !actA && !actB ) break;
+   }
+
+   fa.iters = (uint16_t)((eA & 0x0fff) | (convA ? (1u << 12) : 0u) | (failA ? (1u << 13) : 0u));
+   fb.iters = (uint16_t)((eB & 0x0fff) | (convB ? (1u << 12) : 0u) | (failB ? (1u << 13) : 0u));
+   fa.ok = !failA;
+   fb.ok = !failB;
+   fa.logL = failA ? 0.0f : curA;
+   fb.logL = failB ? 0.0f : curB;
+   fa.E = 0.0f;
+   fb.E = 0.0f;
+   if ( !failA ) fa.E = joint_labels<P, KT, 0, 0, KA>(L, N, va, prm, convA, want_entropy, L.w);
+   if ( !failB ) fb.E = joint_labels<P, KT, KA, 1, KB>(L, N, vb, prm, convB, want_entropy, wb);
+   CENSUS_LAP(KINC_CENSUS_LABELS, ct);
+}
+
 /* --------------------------------------------------------------- KMeans */
 
 __device__ __forceinline__ float diff_norm(float ax, float ay, float bx, float by)
@@ -1764,13 +2192,37 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
       if ( N >= a.min_samples )
       {
          float bestValue = INFINITY;
+         /* register-slot GMM: the fits with 2 and 3 clusters run together;
+          * the second one's labels wait in the tile region past its first
+          * row until its turn in the selection */
+         FitOut f3;
+         bool have3 = false;
+         int8_t* const w3 = (int8_t*)(L.tile + KINC_TILE_STRIDE);
          for ( int Kc = a.min_k; Kc <= a.max_k; ++Kc )
          {
             FitOut f;
+            const int8_t* wsrc = L.w;
             if constexpr ( KMAX > 8 )
             {
                /* run-time K (memory store): max_clusters 9..64 */
                f = (a.clus == 1) ? gmm_fit_dyn(L, N, Kc, a.crit == 1) : kmeans_fit_dyn(a, L, N, Kc);
+            }
+            else if constexpr ( P > 0 )
+            {
+               if ( a.clus == 1 && Kc == 2 && a.max_k >= 3 )
+               {
+                  gmm_fit_joint<P, 2, 3>(L, N, a.crit == 1, f, f3, w3);
+                  have3 = true;
+               }
+               else if ( Kc == 3 && have3 )
+               {
+                  f = f3;
+                  wsrc = w3;
+               }
+               else
+               {
+                  f = fit_dispatch<P, KMAX>(a, L, N, Kc);
+               }
             }
             else
             {
@@ -1790,7 +2242,7 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
             {
                bestK = Kc;
                bestValue = value;
-               copy_back(L, N);
+               copy_back(L, N, wsrc);
             }
          }
       }

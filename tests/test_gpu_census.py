@@ -1,12 +1,15 @@
 """The phase census (kinc_selftest_census): the instrumented build of the
 7-slot, 5-cluster pair kernel gives the product build's bits, and its event
-counters equal what the kernel's own iteration records fix: with 448 valid
+counters equal what the kernel's own iteration records fix.  With 448 valid
 samples per pair (7 full slots of 64 points) and BIC, a fit of K clusters
 that ran `it` E steps and converged (c = 1) or not (c = 0) executes
   E-step slots 7 (it + c), pass A slots 7 it, pass B slots 7 (it - c),
-  chain-sum adds 448 (2 it - c) on 16 active lanes each,
-  exps 7K it (E) + 14K it (pass A) + 2 (it - c) (M step) + 7K c (final E),
-  logs 7 it + 2 (it - c) + 7 c."""
+  exps 7K it (E) + 14K it (pass A) + 7K c (final E), logs 7 it + 7 c,
+and the fits with 1, 4 and 5 clusters each run their own chain sums
+(7 (2 it - c) calls of 64 adds on 16 lanes) and M steps (it - c, 2 exps and
+2 logs each), while the fits with 2 and 3 clusters share theirs: max(it)
+pass-A chain calls on 17 lanes, max(it - c) pass-B calls on 16 lanes and
+M steps."""
 import numpy as np
 import pytest
 import torch
@@ -16,20 +19,27 @@ from kinc_amd import _native, gemgen
 pytestmark = pytest.mark.gpu
 
 
-def expected_counts(iters, K_of_lane):
+def expected_counts(iters):
     it = (iters & 0x0FFF).astype(np.int64)
     c = ((iters >> 12) & 1).astype(np.int64)
-    K = np.broadcast_to(K_of_lane, it.shape).astype(np.int64)
+    K = np.arange(1, 6)[None, :].astype(np.int64)
+    single = [0, 3, 4]
+    ms_single = (it - c)[:, single].sum()
+    ja = np.maximum(it[:, 1], it[:, 2]).sum()
+    jb = np.maximum(it[:, 1] - c[:, 1], it[:, 2] - c[:, 2]).sum()
+    calls_single = (7 * (2 * it - c))[:, single].sum()
+    msteps = ms_single + jb
     return {
         "fits": int(it.size),
         "iters": int(it.sum()),
         "estep_slots": int((7 * (it + c)).sum()),
         "a_slots": int((7 * it).sum()),
         "b_slots": int((7 * (it - c)).sum()),
-        "chain_adds": int((448 * (2 * it - c)).sum()),
-        "chain_calls": int((7 * (2 * it - c)).sum()),
-        "exps": int((7 * K * it + 14 * K * it + 2 * (it - c) + 7 * K * c).sum()),
-        "logs": int((7 * it + 2 * (it - c) + 7 * c).sum()),
+        "chain_calls": int(calls_single + 7 * (ja + jb)),
+        "chain_adds": int(64 * (calls_single + 7 * (ja + jb))),
+        "chain_lanes": int(64 * (16 * (calls_single + 7 * jb) + 17 * 7 * ja)),
+        "exps": int((7 * K * it + 14 * K * it + 7 * K * c).sum() + 2 * msteps),
+        "logs": int((7 * it + 7 * c).sum() + 2 * msteps),
     }
 
 
@@ -55,13 +65,10 @@ def test_census_counts_match_iteration_records_and_results_match():
 
     c = _native.census_dict(ctr.cpu().numpy())
     print({k: v for k, v in c.items() if k != "cycles"}, flush=True)
-    iters = want[3]
-    exp = expected_counts(iters, np.arange(1, 6)[None, :])
+    exp = expected_counts(want[3])
     assert c["pairs"] == n
     for k, v in exp.items():
         assert c[k] == v, (k, c[k], v)
-    # every chain sum runs on 16 lanes (1 + 3K <= 16 chains for K <= 5)
-    assert c["chain_lanes"] == 16 * c["chain_adds"]
     cyc = c["cycles"]
     named = sum(v for k, v in cyc.items() if k != "wave") + c["queue_cycles"]
     assert 0 < named <= cyc["wave"]
