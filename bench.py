@@ -620,10 +620,11 @@ def rmt_leg(gemgen, with_cpu):
 
 
 def valu_calibration():
-    """Issue rate of v_add_f32 streams with 16 one-wave workgroups per CU
-    (4 waves per SIMD, as the pair kernels run): wave64 instructions per
-    second and cycles per instruction per SIMD at the 2.4 GHz peak clock,
-    for full, partial and dependent streams."""
+    """Issue rate of VALU instruction streams with 16 one-wave workgroups per
+    CU (4 waves per SIMD, as the pair kernels run): wave64 instructions per
+    second and cycles per instruction per SIMD at the 2.4 GHz peak clock, for
+    full, partial and dependent streams of each probed opcode; then one wave
+    per SIMD, which shows the dependent-instruction latency."""
     import torch
     from kinc_amd import _native as native
     if not torch.cuda.is_available():
@@ -645,8 +646,19 @@ def valu_calibration():
                      "cycles_per_inst_per_simd": cpi})
         log("valu probe mode %d (%s, %d chains, %d lanes): %.3f ms, %.3g wave insts/s, "
             "%.2f cycles per instruction per SIMD" % (mode, name, chains, lanes, ms, rate, cpi))
-    print(json.dumps({"valu_calibration": rows, "cus": cus, "workgroups": wgs,
-                      "iters": iters, "clock_hz": CLOCK_HZ}), flush=True)
+    # one wave per SIMD (4 per CU): the cycles between one wave's instructions,
+    # dependent (one chain) and independent (eight chains)
+    single = []
+    for mode in (3, 0, 15, 7):
+        chains, lanes, op = native.VALU_PROBE_MODES[mode]
+        ms = native.valu_probe(mode, 2000, cus * 4, 0)
+        cyc = ms * 1e-3 * CLOCK_HZ / (128.0 * 2000)
+        single.append({"mode": mode, "op": native.VALU_PROBE_OPS[op], "chains": chains,
+                       "lanes": lanes, "ms": ms, "cycles_per_inst_per_wave": cyc})
+        log("one wave per SIMD, mode %d (%s, %d chains): %.2f cycles per instruction"
+            % (mode, native.VALU_PROBE_OPS[op], chains, cyc))
+    print(json.dumps({"valu_calibration": rows, "one_wave_per_simd": single, "cus": cus,
+                      "workgroups": wgs, "iters": iters, "clock_hz": CLOCK_HZ}), flush=True)
 
 
 def main():

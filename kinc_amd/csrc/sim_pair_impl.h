@@ -465,10 +465,11 @@ __device__ __forceinline__ float chain_row_var(const float* row, int len, float 
 
 /* Add row `lane` of the staged tile (len values) to acc in index order, for
  * lanes < nch; lanes nch..15 repeat the last row and their sums are unused.
- * A full row is read eight 16-byte groups ahead of the adds (the schedule
- * is pinned: left to itself the compiler waits on every read), so the
- * chain runs at the dependent-add rate instead of one LDS round trip per
- * four adds. */
+ * A full row's reads run one group of two 16-byte reads ahead of the adds
+ * (the schedule is pinned: left to itself the compiler waits on every
+ * read), so the chain runs at the dependent-add rate instead of one LDS
+ * round trip per four adds; a deeper window costs more registers than it
+ * saves (measured). */
 __device__ __forceinline__ float chain_add(const float* tile, int stride, int nch, int len, float acc)
 {
    const int lane = lane_id();
@@ -481,34 +482,34 @@ __device__ __forceinline__ float chain_add(const float* tile, int stride, int nc
       if ( len == TILE )
       {
          const float4* r4 = (const float4*)row;
-         float4 b[4];
+         float4 b[2];
 #pragma unroll
-         for ( int u = 0; u < 4; ++u ) b[u] = r4[u];
+         for ( int u = 0; u < 2; ++u ) b[u] = r4[u];
          /* scheduling groups: 0x100 = LDS reads, 0x002 = VALU */
-         __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+         __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
 #pragma unroll
-         for ( int g = 0; g < TILE / 16; ++g )
+         for ( int g = 0; g < TILE / 8; ++g )
          {
-            float4 nb[4];
-            if ( g + 1 < TILE / 16 )
+            float4 nb[2];
+            if ( g + 1 < TILE / 8 )
             {
 #pragma unroll
-               for ( int u = 0; u < 4; ++u ) nb[u] = r4[4 * (g + 1) + u];
-               __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+               for ( int u = 0; u < 2; ++u ) nb[u] = r4[2 * (g + 1) + u];
+               __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
             }
 #pragma unroll
-            for ( int u = 0; u < 4; ++u )
+            for ( int u = 0; u < 2; ++u )
             {
                acc = acc + b[u].x;
                acc = acc + b[u].y;
                acc = acc + b[u].z;
                acc = acc + b[u].w;
             }
-            __builtin_amdgcn_sched_group_barrier(0x002, 16, 0);
-            if ( g + 1 < TILE / 16 )
+            __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
+            if ( g + 1 < TILE / 8 )
             {
 #pragma unroll
-               for ( int u = 0; u < 4; ++u ) b[u] = nb[u];
+               for ( int u = 0; u < 2; ++u ) b[u] = nb[u];
             }
          }
       }
