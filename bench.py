@@ -53,15 +53,18 @@ WORKLOADS = {
     "E": "KINC similarity, GMM (K=1..5, BIC) + Spearman, minsamp 30; 2000 genes x 5000 samples "
          "FP32, module generator, 10% NaN, 20% of genes near-degenerate (precision stress)",
 }
-# HBM bytes per launch of the dominant pair kernel, from the FETCH_SIZE and
+# HBM bytes per launch of the dominant pair kernel (k_pairs<7,5,4>, the
+# 32,443-pair bin of block 97, one handle), from the FETCH_SIZE and
 # WRITE_SIZE passes (scaled by a copy kernel of known size) recorded in:
 TRAFFIC_SOURCE = "profiles/k_pairs_gmm_spearman_20000x500_counters.md"
-TRAFFIC_BYTES_PER_LAUNCH = 1499.8e6
-# VALU wave64 instructions per second of the same kernel (SQ_INSTS_VALU per
-# dispatch / duration, same record) against the issue ceiling: 1024 SIMDs x
-# 2.4 GHz / 4 cycles per wave64 instruction
-VALU_ISSUE_RATE = 7.396e11
-VALU_ISSUE_CEILING = 1024 * 2.4e9 / 4
+TRAFFIC_BYTES_PER_LAUNCH = 848.9e6
+# VALU wave64 instructions per second of the same dispatch (SQ_INSTS_VALU /
+# its 44.08 ms) against the fastest VALU stream the issue probe measured on
+# the same box (v_add_f32, 4 waves per SIMD; profiles/valu_issue_calibration.md),
+# and the lane utilisation SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)
+VALU_ISSUE_RATE = 8.605e11
+VALU_ISSUE_CEILING = 1.17e12
+VALU_LANE_UTILIZATION = 0.802
 BLOCK = 32768
 
 
@@ -926,6 +929,9 @@ def main():
             "traffic_source": TRAFFIC_SOURCE if profiled else None,
             "valu_issue": ({"insts_per_s": VALU_ISSUE_RATE, "ceiling": VALU_ISSUE_CEILING,
                             "frac": VALU_ISSUE_RATE / VALU_ISSUE_CEILING,
+                            "ceiling_is": "fastest measured VALU stream (v_add_f32, 4 waves "
+                                          "per SIMD)",
+                            "lane_utilization": VALU_LANE_UTILIZATION,
                             "source": TRAFFIC_SOURCE} if profiled else None),
             "whole_job_tflops": whole_tflops,
         }),
