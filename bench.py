@@ -165,7 +165,7 @@ def run_form(native, x, flags, form, blocks, warmup, torch, workers=2, autotune=
         start = blocks[0] * BLOCK
         tuning = engs[0].autotune(start, min(8192, total_pairs - start))
         for e in engs[1:]:
-            e.set_tuning(tuning["w5"], tuning["aux_streams"])
+            e.set_tuning(tuning["w5"], tuning["aux_streams"], tuning["joint45"])
         log("  %s autotune: %s" % (form, tuning))
     MC = p.max_clusters
     nsteps = len(blocks)

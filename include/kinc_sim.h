@@ -187,6 +187,11 @@ int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launches,
  *                5 waves per SIMD (only when the per-wave LDS leaves room for
  *                20 waves per CU); 0: the 4-wave builds
  *   aux_streams  streams the persistent form's bins run on concurrently (1-4)
+ *   joint45      1: the GMM fits with 4 and 5 clusters run in lockstep on the
+ *                register-slot instances of at most 7 slots, sharing their
+ *                chain-sum passes (only when the LDS layout holds their
+ *                29-row tile: about 410 samples or more); 0: one after the
+ *                other.  A handle starts with 1 where it fits.
  * kinc_sim_autotune runs the sample of pairs [start, start+count) (at most
  * one work block) with every candidate configuration, three times each,
  * checks that all of them give identical K and correlations, keeps the
@@ -197,6 +202,7 @@ typedef struct kinc_sim_tuning
 {
    int32_t w5;
    int32_t aux_streams;
+   int32_t joint45;
    int32_t candidates;
    double sample_ms;
 } kinc_sim_tuning;

@@ -133,11 +133,13 @@ def make_params(clusmethod="none", corrmethod="pearson", minexpr=float("-inf"), 
 class Tuning(ctypes.Structure):
     """Mirror of kinc_sim_tuning (include/kinc_sim.h)."""
     _fields_ = [("w5", ctypes.c_int32), ("aux_streams", ctypes.c_int32),
-                ("candidates", ctypes.c_int32), ("sample_ms", ctypes.c_double)]
+                ("joint45", ctypes.c_int32), ("candidates", ctypes.c_int32),
+                ("sample_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {"w5": bool(self.w5), "aux_streams": int(self.aux_streams),
-                "candidates": int(self.candidates), "sample_ms": float(self.sample_ms)}
+                "joint45": bool(self.joint45), "candidates": int(self.candidates),
+                "sample_ms": float(self.sample_ms)}
 
 
 class KincError(RuntimeError):
@@ -256,10 +258,13 @@ class Engine:
         self._check(lib().kinc_sim_get_tuning(self.h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, w5, aux_streams):
+    def set_tuning(self, w5, aux_streams, joint45=None):
+        """Set the launch configuration (kinc_sim_set_tuning); joint45 None
+        keeps the handle's current choice."""
         t = Tuning()
         t.w5 = 1 if w5 else 0
         t.aux_streams = aux_streams
+        t.joint45 = (1 if self.tuning()["joint45"] else 0) if joint45 is None else (1 if joint45 else 0)
         self._check(lib().kinc_sim_set_tuning(self.h, ctypes.byref(t)))
 
     def set_census(self, counters_ptr):

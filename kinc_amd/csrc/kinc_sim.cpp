@@ -163,6 +163,7 @@ struct kinc_sim
     * their 5-waves-per-SIMD builds */
    bool w5 = false;
    bool w5_fits = false;           /* the per-wave LDS allows 20 waves per CU */
+   bool j45_fits = false;          /* the LDS layout holds the lockstep 4+5-cluster tile */
    struct Timed { int P; hipEvent_t a, b; };
    std::vector<Timed> timed;
    std::vector<hipEvent_t> event_pool;
@@ -321,35 +322,68 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
    const size_t tile_bytes = sizeof(float) * (size_t)(1 + 3 * h->kmax) * (size_t)a.tile_stride;
    const size_t sort_bytes = sizeof(float) * 2 * (size_t)a.np2;
 
-   /* per-pair arrays: X, labels, work labels, then sort buffers */
-   size_t off = 0;
-   a.lds_x = (int32_t)off;
-   off = align16(off + sizeof(float) * 2 * (size_t)S);
-   a.lds_lab = (int32_t)off;
-   off = align16(off + (size_t)S);
-   a.lds_w = (int32_t)off;
-   off = align16(off + (size_t)S);
+   a.joint45 = 0;
    if ( h->classes[0] != 0 )
    {
-      /* everything in LDS; the chain tile shares the sort-buffer region.  The
-       * GMM fits with 2 and 3 clusters run together on 17 chain rows, and the
-       * second fit's labels wait past the tile's first row. */
+      /* everything in LDS: labels, work labels, component parameters (nine
+       * sets: the fits with 4 and 5 clusters run together), then the compact
+       * X and after it the region the chain tile shares with the sort
+       * buffers.  The GMM fits with 2 and 3 clusters run together on 17
+       * chain rows of the shared region, and the second fit's labels wait
+       * past the tile's first row; the fits with 4 and 5 clusters (29 rows)
+       * lay their tile over X and the shared region, which therefore follow
+       * each other. */
+      size_t off = 0;
       a.scratch = nullptr;
+      a.lds_lab = (int32_t)off;
+      off = align16(off + (size_t)S);
+      a.lds_w = (int32_t)off;
+      off = align16(off + (size_t)S);
       a.lds_prm = (int32_t)off;
-      off = align16(off + sizeof(float) * 8 * 8);
+      off = align16(off + sizeof(float) * 8 * 9);
+      a.lds_x = (int32_t)off;
+      const size_t x_bytes = align16(sizeof(float) * 2 * (size_t)S);
+      off += x_bytes;
       a.lds_u = (int32_t)off;
       a.lds_tile = a.lds_u;
       const size_t joint_bytes = std::max(sizeof(float) * 17 * (size_t)a.tile_stride,
                                           sizeof(float) * (size_t)a.tile_stride + (size_t)S);
-      off += std::max(std::max(tile_bytes, joint_bytes), sort_bytes);
+      size_t shared = std::max(std::max(tile_bytes, joint_bytes), sort_bytes);
+      if ( a.clus == 1 && a.max_k >= 5 && a.min_k <= 4 )
+      {
+         const size_t j45 = sizeof(float) * 29 * (size_t)a.tile_stride;
+         const size_t need = j45 > x_bytes ? j45 - x_bytes : 0;
+         const size_t cap = (size_t)160 * 1024;
+         if ( need <= shared )
+         {
+            a.joint45 = 1;
+         }
+         else if ( align16(off + shared) * 20 > cap && align16(off + need) * 16 <= cap )
+         {
+            /* a larger shared region costs no residency here: the 5-wave
+             * builds do not fit either way and 16 waves per CU still do */
+            shared = need;
+            a.joint45 = 1;
+         }
+      }
+      off += shared;
+      h->j45_fits = a.joint45 != 0;
       h->lds_bytes = align16(off);
       h->w5_fits = h->lds_bytes * 20 <= (size_t)160 * 1024;
       h->w5 = h->w5_fits;
    }
    else
    {
-      /* memory store: per-pair arrays in a scratch slice per workgroup, the
-       * parameters and the chain tile in LDS */
+      /* memory store: per-pair arrays (X, labels, work labels, sort buffers)
+       * in a scratch slice per workgroup, the parameters and the chain tile
+       * in LDS */
+      size_t off = 0;
+      a.lds_x = (int32_t)off;
+      off = align16(off + sizeof(float) * 2 * (size_t)S);
+      a.lds_lab = (int32_t)off;
+      off = align16(off + (size_t)S);
+      a.lds_w = (int32_t)off;
+      off = align16(off + (size_t)S);
       a.lds_u = (int32_t)off;
       off = align16(off + sort_bytes);
       a.scr_lg = (int32_t)off;
@@ -865,14 +899,16 @@ extern "C" const char* kinc_sim_last_error(const kinc_sim* h)
 /*
  * The launch configurations a handle can run with and that give the same
  * results: the 5-waves-per-SIMD builds of the small register-slot instances
- * (on or off) and how many auxiliary streams the persistent form's bins are
- * spread over.  kinc_sim_autotune times each on a sample of pairs and keeps
- * the fastest.
+ * (on or off), how many auxiliary streams the persistent form's bins are
+ * spread over, and whether the GMM fits with 4 and 5 clusters run in
+ * lockstep (where the LDS layout holds their tile).  kinc_sim_autotune
+ * times each on a sample of pairs and keeps the fastest.
  */
-static int time_config(kinc_sim* h, int64_t start, int count, bool w5, int naux, double* ms)
+static int time_config(kinc_sim* h, int64_t start, int count, bool w5, int naux, bool j45, double* ms)
 {
    h->w5 = w5;
    h->naux = naux;
+   h->base.joint45 = j45 ? 1 : 0;
    hipEvent_t e0 = nullptr, e1 = nullptr;
    HIP_TRY(h, hipEventCreate(&e0));
    int rc = KINC_OK;
@@ -911,55 +947,59 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
    HIP_TRY(h, hipSetDevice(h->device));
    const bool saved_w5 = h->w5;
    const int saved_naux = h->naux;
-   /* candidates; the 5-wave builds only where the LDS leaves room for them */
-   const bool w5s[2] = {false, true};
+   const bool saved_j45 = h->base.joint45 != 0;
+   /* candidates; the 5-wave builds and the lockstep fits only where the LDS
+    * leaves room for them */
+   const bool flags[2] = {false, true};
    const int nauxs[3] = {1, 2, 4};
-   bool best_w5 = saved_w5;
+   bool best_w5 = saved_w5, best_j45 = saved_j45;
    int best_naux = saved_naux;
    double best_ms = 1e30;
    int tried = 0;
    std::vector<int8_t> K0((size_t)count), K1((size_t)count);
    std::vector<float> C0((size_t)count * (size_t)h->p.max_clusters), C1(C0.size());
-   for ( bool w5 : w5s )
+   for ( int c = 0; c < 12 && rc == KINC_OK; ++c )
    {
-      if ( w5 && !h->w5_fits ) continue;
-      for ( int naux : nauxs )
-      {
-         double ms = 0.0;
-         rc = time_config(h, start, (int)count, w5, naux, &ms);
-         if ( rc ) break;
-         /* every configuration must give the same bits */
-         std::vector<int8_t>& K = tried == 0 ? K0 : K1;
-         std::vector<float>& C = tried == 0 ? C0 : C1;
-         if ( hipMemcpy(K.data(), h->d_K, (size_t)count, hipMemcpyDeviceToHost) != hipSuccess ||
-              hipMemcpy(C.data(), h->d_corr, sizeof(float) * C.size(), hipMemcpyDeviceToHost) != hipSuccess )
-         {
-            rc = fail_c(h, KINC_E_DEVICE, "autotune: copying the sample's results failed");
-            break;
-         }
-         if ( tried > 0 && (std::memcmp(K0.data(), K1.data(), K0.size()) != 0 ||
-                            std::memcmp(C0.data(), C1.data(), sizeof(float) * C0.size()) != 0) )
-         {
-            rc = fail_c(h, KINC_E_DEVICE, "autotune: launch configurations gave different results");
-            break;
-         }
-         ++tried;
-         if ( ms < best_ms )
-         {
-            best_ms = ms;
-            best_w5 = w5;
-            best_naux = naux;
-         }
-      }
+      const bool w5 = flags[c / 6];
+      const bool j45 = flags[(c / 3) & 1];
+      const int naux = nauxs[c % 3];
+      if ( (w5 && !h->w5_fits) || (j45 && !h->j45_fits) ) continue;
+      double ms = 0.0;
+      rc = time_config(h, start, (int)count, w5, naux, j45, &ms);
       if ( rc ) break;
+      /* every configuration must give the same bits */
+      std::vector<int8_t>& K = tried == 0 ? K0 : K1;
+      std::vector<float>& C = tried == 0 ? C0 : C1;
+      if ( hipMemcpy(K.data(), h->d_K, (size_t)count, hipMemcpyDeviceToHost) != hipSuccess ||
+           hipMemcpy(C.data(), h->d_corr, sizeof(float) * C.size(), hipMemcpyDeviceToHost) != hipSuccess )
+      {
+         rc = fail_c(h, KINC_E_DEVICE, "autotune: copying the sample's results failed");
+         break;
+      }
+      if ( tried > 0 && (std::memcmp(K0.data(), K1.data(), K0.size()) != 0 ||
+                         std::memcmp(C0.data(), C1.data(), sizeof(float) * C0.size()) != 0) )
+      {
+         rc = fail_c(h, KINC_E_DEVICE, "autotune: launch configurations gave different results");
+         break;
+      }
+      ++tried;
+      if ( ms < best_ms )
+      {
+         best_ms = ms;
+         best_w5 = w5;
+         best_naux = naux;
+         best_j45 = j45;
+      }
    }
    h->w5 = rc ? saved_w5 : best_w5;
    h->naux = rc ? saved_naux : best_naux;
+   h->base.joint45 = (rc ? saved_j45 : best_j45) ? 1 : 0;
    if ( rc ) return rc;
    if ( out )
    {
       out->w5 = h->w5 ? 1 : 0;
       out->aux_streams = h->naux;
+      out->joint45 = h->base.joint45;
       out->candidates = tried;
       out->sample_ms = best_ms;
    }
@@ -985,6 +1025,7 @@ extern "C" int kinc_sim_get_tuning(const kinc_sim* h, kinc_sim_tuning* out)
    if ( !h || !out ) return KINC_E_INVALID;
    out->w5 = h->w5 ? 1 : 0;
    out->aux_streams = h->naux;
+   out->joint45 = h->base.joint45;
    out->candidates = 0;
    out->sample_ms = 0.0;
    return KINC_OK;
@@ -995,8 +1036,10 @@ extern "C" int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t)
    if ( !h || !t ) return KINC_E_INVALID;
    if ( t->aux_streams < 1 || t->aux_streams > kinc_sim::kAux ) return fail_c(h, KINC_E_INVALID, "aux_streams must lie in [1, 4]");
    if ( t->w5 && !h->w5_fits ) return fail_c(h, KINC_E_UNSUPPORTED, "the per-wave LDS leaves no room for 5 waves per SIMD");
+   if ( t->joint45 && !h->j45_fits ) return fail_c(h, KINC_E_UNSUPPORTED, "the LDS layout holds no tile for the lockstep 4- and 5-cluster fits");
    h->w5 = t->w5 != 0;
    h->naux = t->aux_streams;
+   h->base.joint45 = t->joint45 ? 1 : 0;
    return KINC_OK;
 }
 

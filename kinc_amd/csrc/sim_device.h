@@ -64,6 +64,10 @@ struct SimArgs
    int32_t pre_out;
    int32_t post_out;
    int32_t pearson_fp64;     /* Pearson sums and quotient in double */
+   int32_t joint45;          /* GMM fits with 4 and 5 clusters in lockstep (register-
+                                slot instances of at most 7 slots): the host sets it
+                                when X and the shared LDS region after it hold their
+                                29-row chain tile */
 
    const int2* pairs;        /* explicit (x, y) of each block-relative pair, or null
                                 for linear indices from block_start */
@@ -90,7 +94,8 @@ struct SimArgs
    int32_t lds_x;            /* float2[S]  compact X, then the stale-read view */
    int32_t lds_lab;          /* int8[S]    full-index labels */
    int32_t lds_w;            /* int8[S]    work labels of the current fit */
-   int32_t lds_prm;          /* float[8][8] component parameters */
+   int32_t lds_prm;          /* float[9][8] component parameters (register-slot
+                                instances; [8][8] on the memory-store instance) */
    int32_t lds_tile;         /* chain-term tile */
    int32_t lds_u;            /* sort buffers / sequence scratch */
    char* scratch;            /* memory-store instance: slices of scratch_stride bytes */

@@ -465,11 +465,14 @@ __device__ __forceinline__ float chain_row_var(const float* row, int len, float 
 
 /* Add row `lane` of the staged tile (len values) to acc in index order, for
  * lanes < nch; lanes nch..15 repeat the last row and their sums are unused.
- * A full row's reads run one group of two 16-byte reads ahead of the adds
+ * A full row's reads run AHEAD groups of 16-byte reads ahead of the adds
  * (the schedule is pinned: left to itself the compiler waits on every
  * read), so the chain runs at the dependent-add rate instead of one LDS
- * round trip per four adds; a deeper window costs more registers than it
- * saves (measured). */
+ * round trip per four adds; two reads ahead was the fastest where the
+ * registers allow (a deeper window costs more registers than it saves,
+ * measured), one where the caller keeps more values live (the lockstep
+ * 4- and 5-cluster fits). */
+template <int AHEAD = 2>
 __device__ __forceinline__ float chain_add(const float* tile, int stride, int nch, int len, float acc)
 {
    const int lane = lane_id();
@@ -481,35 +484,36 @@ __device__ __forceinline__ float chain_add(const float* tile, int stride, int nc
       const float* row = tile + (lane < nch ? lane : nch - 1) * stride;
       if ( len == TILE )
       {
+         constexpr int G = 4 * AHEAD;            /* values per group */
          const float4* r4 = (const float4*)row;
-         float4 b[2];
+         float4 b[AHEAD];
 #pragma unroll
-         for ( int u = 0; u < 2; ++u ) b[u] = r4[u];
+         for ( int u = 0; u < AHEAD; ++u ) b[u] = r4[u];
          /* scheduling groups: 0x100 = LDS reads, 0x002 = VALU */
-         __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+         __builtin_amdgcn_sched_group_barrier(0x100, AHEAD, 0);
 #pragma unroll
-         for ( int g = 0; g < TILE / 8; ++g )
+         for ( int g = 0; g < TILE / G; ++g )
          {
-            float4 nb[2];
-            if ( g + 1 < TILE / 8 )
+            float4 nb[AHEAD];
+            if ( g + 1 < TILE / G )
             {
 #pragma unroll
-               for ( int u = 0; u < 2; ++u ) nb[u] = r4[2 * (g + 1) + u];
-               __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+               for ( int u = 0; u < AHEAD; ++u ) nb[u] = r4[AHEAD * (g + 1) + u];
+               __builtin_amdgcn_sched_group_barrier(0x100, AHEAD, 0);
             }
 #pragma unroll
-            for ( int u = 0; u < 2; ++u )
+            for ( int u = 0; u < AHEAD; ++u )
             {
                acc = acc + b[u].x;
                acc = acc + b[u].y;
                acc = acc + b[u].z;
                acc = acc + b[u].w;
             }
-            __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
-            if ( g + 1 < TILE / 8 )
+            __builtin_amdgcn_sched_group_barrier(0x002, G, 0);
+            if ( g + 1 < TILE / G )
             {
 #pragma unroll
-               for ( int u = 0; u < 2; ++u ) b[u] = nb[u];
+               for ( int u = 0; u < AHEAD; ++u ) b[u] = nb[u];
             }
          }
       }
@@ -1120,7 +1124,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
 /* ------------------------------------------------------ GMM, two fits */
 
 /*
- * Two GMM fits of one pair (KA and KB components, KA + KB <= 5) run in
+ * Two GMM fits of one pair (KA and KB components, KA + KB <= 9) run in
  * lockstep: the same EM arithmetic per fit as gmm_fit, in the same order per
  * value, but one chain tile holds both fits' rows, so each pass's sequential
  * chain sums (64 dependent adds per slot, the latency-bound part of an
@@ -1129,6 +1133,16 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
  * are skipped, its parameters frozen) while the other runs on.  Component k
  * of the first fit is parameter set k; component k of the second is set
  * KA + k.  Labels of the first fit go to L.w, of the second to wb.
+ *
+ * The chain tile of up to 17 rows (KA + KB <= 5) is the shared tile region.
+ * A larger pair of fits (4 and 5 clusters: 29 pass-A rows) lays its tile
+ * over the compact X and the shared region after it, which the host places
+ * next to each other: X is in registers for the whole fit and the tile is
+ * only written after the initial draws have read X, and X[0, N) is written
+ * back from the registers before the fit returns (X[N, S) is rewritten by
+ * every later reader that needs it, fill_stale).  wb and the entropy chain's
+ * row lie in the shared region past X, which the tile no longer uses once
+ * the EM loop has ended.
  */
 template <int P, int KT>
 struct JointStore
@@ -1299,13 +1313,14 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
                                               FitOut& fb, int8_t* wb)
 {
    constexpr int KT = KA + KB;
-   static_assert(KT <= 5, "two fits share at most five components");
+   static_assert(KT <= 9, "two fits share at most nine components");
    constexpr int STR = KINC_TILE_STRIDE;
    constexpr int RB = 1 + 3 * KA;          /* first pass-A row of the second fit */
    constexpr int NCA = RB + 1 + 3 * KB;    /* pass-A chains */
    const int lane = lane_id();
    const int nslots = (N + WAVE - 1) / WAVE;
-   float* tile = L.tile;
+   /* more than 17 rows: the tile over X and the shared region after it */
+   float* tile = (NCA > 17) ? (float*)L.X : L.tile;
    float* prm = L.prm;
    CENSUS_T(ct);
    CENSUS_COUNT(KINC_CENSUS_FITS, 2);
@@ -1395,7 +1410,7 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
             /* opaque to loop-invariant hoisting, as in gmm_fit */
             asm volatile("" : "+s"(len));
             CENSUS_LAP(KINC_CENSUS_A_TERMS, ct);
-            acc = chain_add(tile, STR, NCA, len, acc);
+            acc = chain_add<(NCA > 17 ? 1 : 2)>(tile, STR, NCA, len, acc);
             wave_sync(false);
             CENSUS_LAP(KINC_CENSUS_A_CHAIN, ct);
          }
@@ -1498,7 +1513,7 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
             len = len < TILE ? len : TILE;
             asm volatile("" : "+s"(len));
             CENSUS_LAP(KINC_CENSUS_B_TERMS, ct);
-            accB = chain_add(tile, STR, 3 * KT, len, accB);
+            accB = chain_add<(NCA > 17 ? 1 : 2)>(tile, STR, 3 * KT, len, accB);
             wave_sync(false);
             CENSUS_LAP(KINC_CENSUS_B_CHAIN, ct);
          }
@@ -1542,6 +1557,17 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
    fb.E = 0.0f;
    if ( !failA ) fa.E = joint_labels<P, KT, 0, 0, KA>(L, N, va, prm, convA, want_entropy, L.w);
    if ( !failB ) fb.E = joint_labels<P, KT, KA, 1, KB>(L, N, vb, prm, convB, want_entropy, wb);
+   if constexpr ( NCA > 17 )
+   {
+      /* the tile covered X: write the pair's points back */
+#pragma unroll
+      for ( int m = 0; m < P; ++m )
+      {
+         const int i = lane + m * WAVE;
+         if ( m < nslots && i < N ) L.X[i] = make_float2(js.x0[m], js.x1[m]);
+      }
+      wave_sync(false);
+   }
    CENSUS_LAP(KINC_CENSUS_LABELS, ct);
 }
 
@@ -2163,6 +2189,15 @@ __device__ __forceinline__ FitOut fit_dispatch(const SimArgs& a, const Lds& L, i
    return f;
 }
 
+/* The fits with 4 and 5 clusters run in lockstep (a.joint45) on register-
+ * slot instances of at most 7 slots built for 4 waves per SIMD: their nine
+ * components' responsibilities take 63 of the 128 VGPRs. */
+template <int P>
+__device__ __forceinline__ constexpr bool joint45_built()
+{
+   return P > 0 && P <= 7 && KINC_PAIR_WAVES == 4;
+}
+
 template <int P, int KMAX>
 __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
 {
@@ -2193,12 +2228,13 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
       if ( N >= a.min_samples )
       {
          float bestValue = INFINITY;
-         /* register-slot GMM: the fits with 2 and 3 clusters run together;
-          * the second one's labels wait in the tile region past its first
-          * row until its turn in the selection */
-         FitOut f3;
-         bool have3 = false;
-         int8_t* const w3 = (int8_t*)(L.tile + KINC_TILE_STRIDE);
+         /* register-slot GMM: the fits with 2 and 3 clusters run together,
+          * and (a.joint45, at most 7 slots) so do those with 4 and 5; the
+          * second fit's labels wait in the tile region past its first row
+          * until its turn in the selection */
+         FitOut fs;
+         int pend = 0;   /* cluster count of the joint fit whose result waits in fs */
+         int8_t* const ws = (int8_t*)(L.tile + KINC_TILE_STRIDE);
          for ( int Kc = a.min_k; Kc <= a.max_k; ++Kc )
          {
             FitOut f;
@@ -2210,15 +2246,21 @@ __device__ __forceinline__ void process_pair(const SimArgs& a, int q)
             }
             else if constexpr ( P > 0 )
             {
-               if ( a.clus == 1 && Kc == 2 && a.max_k >= 3 )
+               if ( Kc == pend )
                {
-                  gmm_fit_joint<P, 2, 3>(L, N, a.crit == 1, f, f3, w3);
-                  have3 = true;
+                  f = fs;
+                  wsrc = ws;
+                  pend = 0;
                }
-               else if ( Kc == 3 && have3 )
+               else if ( a.clus == 1 && Kc == 2 && a.max_k >= 3 )
                {
-                  f = f3;
-                  wsrc = w3;
+                  gmm_fit_joint<P, 2, 3>(L, N, a.crit == 1, f, fs, ws);
+                  pend = 3;
+               }
+               else if ( joint45_built<P>() && a.joint45 && a.clus == 1 && Kc == 4 && a.max_k >= 5 )
+               {
+                  if constexpr ( joint45_built<P>() ) gmm_fit_joint<P, 4, 5>(L, N, a.crit == 1, f, fs, ws);
+                  pend = 5;
                }
                else
                {

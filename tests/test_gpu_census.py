@@ -5,11 +5,11 @@ samples per pair (7 full slots of 64 points) and BIC, a fit of K clusters
 that ran `it` E steps and converged (c = 1) or not (c = 0) executes
   E-step slots 7 (it + c), pass A slots 7 it, pass B slots 7 (it - c),
   exps 7K it (E) + 14K it (pass A) + 7K c (final E), logs 7 it + 7 c,
-and the fits with 1, 4 and 5 clusters each run their own chain sums
-(7 (2 it - c) calls of 64 adds on 16 lanes) and M steps (it - c, 2 exps and
-2 logs each), while the fits with 2 and 3 clusters share theirs: max(it)
-pass-A chain calls on 17 lanes, max(it - c) pass-B calls on 16 lanes and
-M steps."""
+and the fit with 1 cluster runs its own chain sums (7 (2 it - c) calls of
+64 adds on 16 lanes) and M steps (it - c, 2 exps and 2 logs each), while
+the fits with 2 and 3 clusters share theirs (max(it) pass-A chain calls on
+17 lanes, max(it - c) pass-B calls on 16 lanes, and M steps), and so do the
+fits with 4 and 5 clusters (pass A on 29 lanes, pass B on 27)."""
 import numpy as np
 import pytest
 import torch
@@ -23,21 +23,28 @@ def expected_counts(iters):
     it = (iters & 0x0FFF).astype(np.int64)
     c = ((iters >> 12) & 1).astype(np.int64)
     K = np.arange(1, 6)[None, :].astype(np.int64)
-    single = [0, 3, 4]
+    single = [0]
     ms_single = (it - c)[:, single].sum()
-    ja = np.maximum(it[:, 1], it[:, 2]).sum()
-    jb = np.maximum(it[:, 1] - c[:, 1], it[:, 2] - c[:, 2]).sum()
     calls_single = (7 * (2 * it - c))[:, single].sum()
-    msteps = ms_single + jb
+    calls = calls_single
+    lanes = 16 * calls_single
+    msteps = ms_single
+    # the joint pairs of fits: (column a, column b, pass-A lanes, pass-B lanes)
+    for ka, kb, la, lb in ((1, 2, 17, 16), (3, 4, 29, 27)):
+        ja = np.maximum(it[:, ka], it[:, kb]).sum()
+        jb = np.maximum(it[:, ka] - c[:, ka], it[:, kb] - c[:, kb]).sum()
+        calls += 7 * (ja + jb)
+        lanes += 7 * (la * ja + lb * jb)
+        msteps += jb
     return {
         "fits": int(it.size),
         "iters": int(it.sum()),
         "estep_slots": int((7 * (it + c)).sum()),
         "a_slots": int((7 * it).sum()),
         "b_slots": int((7 * (it - c)).sum()),
-        "chain_calls": int(calls_single + 7 * (ja + jb)),
-        "chain_adds": int(64 * (calls_single + 7 * (ja + jb))),
-        "chain_lanes": int(64 * (16 * (calls_single + 7 * jb) + 17 * 7 * ja)),
+        "chain_calls": int(calls),
+        "chain_adds": int(64 * calls),
+        "chain_lanes": int(64 * lanes),
         "exps": int((7 * K * it + 14 * K * it + 7 * K * c).sum() + 2 * msteps),
         "logs": int((7 * it + 7 * c).sum() + 2 * msteps),
     }

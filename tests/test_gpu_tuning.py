@@ -1,6 +1,7 @@
 """The launch-configuration autotuner (kinc_sim_autotune): every
 configuration a handle can take (5-wave builds of the small bins on or off,
-1, 2 or 4 auxiliary streams for the bins) gives the same bits, equal to the
+1, 2 or 4 auxiliary streams for the bins, the GMM fits with 4 and 5 clusters
+in lockstep or one after the other) gives the same bits, equal to the
 oracle's, and the tuner keeps one of them."""
 import numpy as np
 import pytest
@@ -43,8 +44,51 @@ def test_five_wave_builds_refused_where_lds_leaves_no_room():
     eng = _native.Engine(x, _native.make_params(clusmethod="gmm", corrmethod="spearman"))
     try:
         t = eng.autotune(0, 150)
-        assert t["w5"] is False and t["candidates"] == 3
+        # 900 samples: no 5-wave builds; the lockstep fits on or off
+        assert t["w5"] is False and t["candidates"] == 6
         with pytest.raises(_native.KincError):
             eng.set_tuning(True, 4)
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("S,nan,flags", [
+    # about 406 valid samples: the 7-slot instance
+    (480, 0.08, dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True,
+                     postout=True)),
+    # about 380: the 6-slot instance
+    (448, 0.08, dict(clusmethod="gmm", corrmethod="pearson", crit="ICL", preout=True,
+                     postout=False)),
+    # up to 8 clusters (fits 2+3 and 4+5 in lockstep, 6 alone), about 200
+    # valid samples: the 4-slot instance of the 8-cluster kernel
+    (480, 0.35, dict(clusmethod="gmm", corrmethod="spearman", crit="ICL", preout=False,
+                     postout=True, minclus=2, maxclus=6)),
+])
+def test_lockstep_four_and_five_cluster_fits(S, nan, flags):
+    """The fits with 4 and 5 clusters in lockstep (their chain tile over X)
+    give the bits of the fits one after the other and of the oracle."""
+    x = gemgen.modules(120, S, seed=33, nan_rate=nan)
+    p = _native.make_params(**flags)
+    eng = _native.Engine(x, p)
+    try:
+        assert eng.tuning()["joint45"] is True
+        got = eng.execute(0, 3000)
+        eng.set_tuning(False, 4, joint45=False)
+        assert eng.tuning()["joint45"] is False
+        assert_block_equal(eng.execute(0, 3000), got, S, "joint45 off")
+        eng.set_tuning(False, 2, joint45=True)
+        assert_block_equal(eng.execute(0, 3000), got, S, "joint45 on again")
+    finally:
+        eng.close()
+    assert_block_equal(got, oracle.execute_block(p, x, 0, 3000), S, "oracle")
+
+
+def test_lockstep_fits_refused_where_lds_leaves_no_room():
+    x = gemgen.modules(20, 256, seed=34, nan_rate=0.1)
+    eng = _native.Engine(x, _native.make_params(clusmethod="gmm", corrmethod="spearman"))
+    try:
+        assert eng.tuning()["joint45"] is False
+        with pytest.raises(_native.KincError):
+            eng.set_tuning(False, 4, joint45=True)
     finally:
         eng.close()
