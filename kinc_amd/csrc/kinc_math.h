@@ -159,19 +159,25 @@ KINC_HD float kinc_expf(float x)
    /* Outside [-104, 89] the result is 0 or +inf; clamping there lets the
     * final ldexp produce them with no extra case (exp(-104) < 2^-150 rounds
     * to 0, exp(89) overflows). */
+#if defined(__HIP_DEVICE_COMPILE__)
+   /* NaN-propagating clamp (v_minimum3_f32 / v_maximum3_f32): NaN stays NaN
+    * through the reduction */
+   return kinc_expf_reduced(__builtin_elementwise_maximum(__builtin_elementwise_minimum(x, 89.0f), -104.0f));
+#else
    const float r = kinc_expf_reduced(kinc_clampf(x, -104.0f, 89.0f));
    return (x != x) ? x : r;
+#endif
 }
 
 /* kinc_expf for x <= 0 or NaN, the arguments v - max(v) of a log-sum-exp.
- * On the device: the lower clamp alone, as a select that a NaN passes
- * through; the NaN then stays NaN through the reduction (kinc_f2i maps it
- * to 0, and ldexp keeps the NaN).  The same value
- * as kinc_expf for such x, which the host evaluates directly. */
+ * On the device: the lower clamp alone, as a NaN-propagating maximum
+ * (v_maximum3_f32); the NaN then stays NaN through the reduction (kinc_f2i
+ * maps it to 0, and ldexp keeps the NaN).  The same value as kinc_expf for
+ * such x, which the host evaluates directly. */
 KINC_HD float kinc_expf_le0(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-   return kinc_expf_reduced(x < -104.0f ? -104.0f : x);
+   return kinc_expf_reduced(__builtin_elementwise_maximum(x, -104.0f));
 #else
    return kinc_expf(x);
 #endif
