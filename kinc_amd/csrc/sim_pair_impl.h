@@ -92,32 +92,27 @@ __device__ __forceinline__ int bcast_i(int v, int src)
    return __builtin_amdgcn_readlane(v, src);
 }
 
-/* One step of a DPP maximum: v and the value of the lane that CTRL selects,
- * in the rows of ROW_MASK (the other lanes keep v). */
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_max(float v)
-{
-   const float o = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                              CTRL, ROW_MASK, 0xF, false));
-   return (o > v) ? o : v;
-}
-
 /*
  * NaN-skipping max in the reference's `if (v > max) max = v` sense, of
- * values that are never NaN, so the order of comparison does not matter:
- * quads by DPP quad permutations, then half rows and rows by mirrors, then
- * across rows with row_bcast:15 and row_bcast:31 (lane 63 ends with the
- * maximum), read back to every lane.  VALU moves with a few cycles of
- * latency instead of six LDS permutes.
+ * values that are never NaN (per-lane maxima taken with fmaxf, which skips
+ * NaN, from -inf), so the order of comparison does not matter, and whose
+ * zero sign no caller reads: quads by DPP quad permutations, then half rows
+ * and rows by mirrors, then across rows with row_bcast:15 and row_bcast:31
+ * (lane 63 ends with the maximum), read back to every lane.  Each step is
+ * one v_max_f32 with a DPP source instead of a DPP move, a compare and a
+ * select, and no step goes through LDS.
  */
 __device__ __forceinline__ float wave_max(float v)
 {
-   v = dpp_max<0xB1, 0xF>(v);    /* quad_perm [1,0,3,2] */
-   v = dpp_max<0x4E, 0xF>(v);    /* quad_perm [2,3,0,1] */
-   v = dpp_max<0x141, 0xF>(v);   /* row_half_mirror */
-   v = dpp_max<0x140, 0xF>(v);   /* row_mirror */
-   v = dpp_max<0x142, 0xA>(v);   /* row_bcast:15 into rows 1 and 3 */
-   v = dpp_max<0x143, 0xC>(v);   /* row_bcast:31 into rows 2 and 3 */
+   /* each step one v_max_f32 with a DPP source, in place (lanes of rows
+    * outside the row mask are not written and keep v); the s_nop gives the
+    * two wait states a DPP read needs after the VALU write of its source */
+   asm volatile("s_nop 1\n\tv_max_f32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf" : "+v"(v));
+   asm volatile("s_nop 1\n\tv_max_f32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf" : "+v"(v));
+   asm volatile("s_nop 1\n\tv_max_f32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf" : "+v"(v));
+   asm volatile("s_nop 1\n\tv_max_f32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf" : "+v"(v));
+   asm volatile("s_nop 1\n\tv_max_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf" : "+v"(v));
+   asm volatile("s_nop 1\n\tv_max_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf" : "+v"(v));
    return bcast(v, 63);
 }
 
