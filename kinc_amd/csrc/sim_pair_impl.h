@@ -843,10 +843,10 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       CENSUS_COUNT(KINC_CENSUS_EXPS, K * nslots);
       CENSUS_COUNT(KINC_CENSUS_LOGS, nslots);
 
-      /* max over the points of each component's log responsibility; only
-       * the last slot holds lanes past N.  fmaxf as in the E step: v is never
-       * NaN, and the sign of a zero maximum does not reach l - gmax or
-       * gmax + log(sum) */
+      /* max over the points of each component's log responsibility
+       * (pairwise_gmm.cpp:200-207); only the last slot holds lanes past N.
+       * fmaxf skips a NaN as the reference's `>` does, and the sign of a
+       * zero maximum does not reach l - gmax or gmax + log(sum) */
       float gmax[K];
 #pragma unroll
       for ( int k = 0; k < K; ++k )
