@@ -165,7 +165,7 @@ def run_form(native, x, flags, form, blocks, warmup, torch, workers=2, autotune=
         start = blocks[0] * BLOCK
         tuning = engs[0].autotune(start, min(8192, total_pairs - start))
         for e in engs[1:]:
-            e.set_tuning(tuning["w5"], tuning["aux_streams"], tuning["joint45"])
+            e.set_tuning(tuning["pair_waves"], tuning["aux_streams"], tuning["joint45"])
         log("  %s autotune: %s" % (form, tuning))
     MC = p.max_clusters
     nsteps = len(blocks)
@@ -227,7 +227,7 @@ def census_leg(native, x, flags, blocks, out):
     total = native.pair_count(G)
     p = native.make_params(form="persistent", **flags)
     e = native.Engine(x, p)
-    e.set_tuning(False, 4)
+    e.set_tuning(4, 4)
     ctr = torch.zeros(native.CENSUS_COUNTERS, dtype=torch.int64, device="cuda:0")
     try:
         e.set_census(ctr.data_ptr())

@@ -183,13 +183,19 @@ int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launches,
  * Launch configuration of a handle (instrumentation and tuning; the
  * reference's only performance knob is --ksize, src/core/similarity_input.cpp:
  * 241-251).  Every configuration gives the same results.
- *   w5           1: the bins of at most 6 register slots run their builds for
- *                5 waves per SIMD (only when the per-wave LDS leaves room for
- *                20 waves per CU); 0: the 4-wave builds
+ *   pair_waves   waves per SIMD the register-slot bins' kernel builds are
+ *                made for: 4; 5 (the instances of at most 6 slots, only when
+ *                the per-wave LDS leaves room for 20 waves per CU); 3 (the
+ *                instances of 5 slots or more, only when the per-wave LDS
+ *                allows at most 12 waves per CU: the same residency with 168
+ *                VGPRs per wave instead of 128).  A handle starts with 5
+ *                where it fits, else 3 where the LDS caps residency at 12,
+ *                else 4
  *   aux_streams  streams the persistent form's bins run on concurrently (1-4)
  *   joint45      1: the GMM fits with 4 and 5 clusters run in lockstep on the
- *                register-slot instances of at most 7 slots, sharing their
- *                chain-sum passes (only when the LDS layout holds their
+ *                register-slot instances whose VGPR budget holds them (at
+ *                most 7 slots at 4 waves per SIMD, 10 at 3), sharing
+ *                their chain-sum passes (only when the LDS layout holds their
  *                29-row tile: about 410 samples or more); 0: one after the
  *                other.  A handle starts with 1 where it fits.
  * kinc_sim_autotune runs the sample of pairs [start, start+count) (at most
@@ -200,7 +206,7 @@ int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launches,
  */
 typedef struct kinc_sim_tuning
 {
-   int32_t w5;
+   int32_t pair_waves;
    int32_t aux_streams;
    int32_t joint45;
    int32_t candidates;

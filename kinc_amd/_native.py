@@ -132,12 +132,12 @@ def make_params(clusmethod="none", corrmethod="pearson", minexpr=float("-inf"), 
 
 class Tuning(ctypes.Structure):
     """Mirror of kinc_sim_tuning (include/kinc_sim.h)."""
-    _fields_ = [("w5", ctypes.c_int32), ("aux_streams", ctypes.c_int32),
+    _fields_ = [("pair_waves", ctypes.c_int32), ("aux_streams", ctypes.c_int32),
                 ("joint45", ctypes.c_int32), ("candidates", ctypes.c_int32),
                 ("sample_ms", ctypes.c_double)]
 
     def as_dict(self):
-        return {"w5": bool(self.w5), "aux_streams": int(self.aux_streams),
+        return {"pair_waves": int(self.pair_waves), "aux_streams": int(self.aux_streams),
                 "joint45": bool(self.joint45), "candidates": int(self.candidates),
                 "sample_ms": float(self.sample_ms)}
 
@@ -258,11 +258,11 @@ class Engine:
         self._check(lib().kinc_sim_get_tuning(self.h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, w5, aux_streams, joint45=None):
+    def set_tuning(self, pair_waves, aux_streams, joint45=None):
         """Set the launch configuration (kinc_sim_set_tuning); joint45 None
         keeps the handle's current choice."""
         t = Tuning()
-        t.w5 = 1 if w5 else 0
+        t.pair_waves = pair_waves
         t.aux_streams = aux_streams
         t.joint45 = (1 if self.tuning()["joint45"] else 0) if joint45 is None else (1 if joint45 else 0)
         self._check(lib().kinc_sim_set_tuning(self.h, ctypes.byref(t)))

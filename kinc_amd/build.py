@@ -54,6 +54,12 @@ INSTANCES = ([(5, p) for p in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16)]
 # 500 samples, where LDS allows 16, mixing them into the concurrent bins
 # cost time
 INSTANCES_W5 = [(5, p) for p in (1, 2, 3, 4, 5, 6)] + [(8, p) for p in (2, 4)]
+# the instances of 5 slots or more are built a third time for 3 waves per
+# SIMD (168 VGPRs); a handle uses those when its per-wave LDS allows at most
+# 12 waves per CU anyway (about 640 samples or more with outlier removal or
+# Spearman, about 800 without), where the 4-wave builds would spill for no
+# extra residency
+INSTANCES_WIDE = [(5, p) for p in (5, 6, 7, 8, 10, 12, 14, 16)] + [(8, 4), (8, 8)]
 # the phase-census build (per-phase cycle and event counters, profiling only)
 # of the instance that carries the 20,000 x 500 GMM workload
 CENSUS_INSTANCES = [(5, 7)]
@@ -70,18 +76,27 @@ def _compile(src, defines=(), tag=None):
     return out
 
 
+def _inst_job(k, p, waves, census=False):
+    """Compile job of one pair-kernel instance, with its rough relative cost
+    (register slots x clusters; the memory-store instance counts as 4 slots)."""
+    defines = ["KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p, "KINC_PAIR_WAVES=%d" % waves]
+    tag = "pairs_k%d_p%d" % (k, p)
+    if census:
+        defines.append("KINC_CENSUS=1")
+        tag = "pairs_census_k%d_p%d" % (k, p)
+    elif waves != PAIR_WAVES:
+        tag += "_w%d" % waves
+    return (k * (p or 4), ("sim_pairs_inst.hip", tuple(defines), tag))
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    inst = sorted(INSTANCES, key=lambda kp: -(kp[0] * (kp[1] or 4)))  # longest first
-    jobs = [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p,
-                                    "KINC_PAIR_WAVES=%d" % PAIR_WAVES),
-             "pairs_k%d_p%d" % (k, p)) for k, p in inst]
-    jobs += [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p,
-                                     "KINC_PAIR_WAVES=5"),
-              "pairs_k%d_p%d_w5" % (k, p)) for k, p in INSTANCES_W5]
-    jobs += [("sim_pairs_inst.hip", ("KINC_INST_KMAX=%d" % k, "KINC_INST_P=%d" % p,
-                                     "KINC_PAIR_WAVES=%d" % PAIR_WAVES, "KINC_CENSUS=1"),
-              "pairs_census_k%d_p%d" % (k, p)) for k, p in CENSUS_INSTANCES]
+    inst = [_inst_job(k, p, PAIR_WAVES) for k, p in INSTANCES]
+    inst += [_inst_job(k, p, 5) for k, p in INSTANCES_W5]
+    inst += [_inst_job(k, p, 3) for k, p in INSTANCES_WIDE]
+    inst += [_inst_job(k, p, PAIR_WAVES, census=True) for k, p in CENSUS_INSTANCES]
+    # longest first, so the pool does not end on one large instance
+    jobs = [j for _, j in sorted(inst, key=lambda cj: -cj[0])]
     jobs += [(src, ("KINC_PAIR_WAVES=%d" % PAIR_WAVES,), None) for src in SOURCES]
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with ThreadPoolExecutor(max_workers=workers) as ex:

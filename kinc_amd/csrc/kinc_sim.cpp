@@ -30,6 +30,10 @@
    X(5, 12) X(5, 14) X(5, 16) X(8, 0) X(8, 2) X(8, 4) X(8, 8) X(64, 0)
 /* small instances also built for 5 waves per SIMD */
 #define KINC_FOR_EACH_INSTANCE_W5(X) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(8, 2) X(8, 4)
+/* the register-slot instances that spill at 4 waves per SIMD, also built for
+ * 3 (168 VGPRs) */
+#define KINC_FOR_EACH_INSTANCE_WIDE(X)                                              \
+   X(5, 5) X(5, 6) X(5, 7) X(5, 8) X(5, 10) X(5, 12) X(5, 14) X(5, 16) X(8, 4) X(8, 8)
 
 extern "C" {
 #define KINC_DECL(K, P) const void* kinc_pair_kernel_k##K##_p##P();
@@ -37,6 +41,9 @@ KINC_FOR_EACH_INSTANCE(KINC_DECL)
 #undef KINC_DECL
 #define KINC_DECL(K, P) const void* kinc_pair_kernel_k##K##_p##P##_w5();
 KINC_FOR_EACH_INSTANCE_W5(KINC_DECL)
+#undef KINC_DECL
+#define KINC_DECL(K, P) const void* kinc_pair_kernel_k##K##_p##P##_w3();
+KINC_FOR_EACH_INSTANCE_WIDE(KINC_DECL)
 #undef KINC_DECL
 const void* kinc_kernel_count_valid();
 const void* kinc_kernel_chunk_max();
@@ -85,15 +92,23 @@ int slot_classes(int S, int kmax, int* out)
    return c;
 }
 
-/* The pair kernel for (P, KMAX); one instance serves both forms.  With w5,
- * the 5-waves-per-SIMD build where one exists. */
-const void* kinc_select_pair_kernel(int P, int kmax, bool w5 = false)
+/* The pair kernel for (P, KMAX); one instance serves both forms.  waves
+ * picks the build for that many waves per SIMD where one exists (5: the small
+ * instances; 3: the instances of 5 slots or more), else the 4-wave one. */
+const void* kinc_select_pair_kernel(int P, int kmax, int waves = 4)
 {
 #define KINC_PICK(K, PP)                                                      \
    if ( kmax == K && P == PP ) return kinc_pair_kernel_k##K##_p##PP##_w5();
-   if ( w5 )
+   if ( waves == 5 )
    {
       KINC_FOR_EACH_INSTANCE_W5(KINC_PICK)
+   }
+#undef KINC_PICK
+#define KINC_PICK(K, PP)                                                      \
+   if ( kmax == K && P == PP ) return kinc_pair_kernel_k##K##_p##PP##_w3();
+   if ( waves == 3 )
+   {
+      KINC_FOR_EACH_INSTANCE_WIDE(KINC_PICK)
    }
 #undef KINC_PICK
 #define KINC_PICK(K, PP)                                                      \
@@ -159,10 +174,13 @@ struct kinc_sim
    /* kernel timing */
    bool profiling = false;
    bool capturing = false;         /* between kinc_sim_graph_begin and _end */
-   /* the per-wave LDS leaves room for 20 waves per CU: small instances run
-    * their 5-waves-per-SIMD builds */
-   bool w5 = false;
+   /* waves per SIMD of the pair-kernel builds the register-slot bins run:
+    * 4, or 5 for the small instances where the per-wave LDS leaves room for
+    * 20 waves per CU, or 3 where it allows at most 12 (the same residency
+    * with 168 VGPRs per wave) */
+   int waves = 4;
    bool w5_fits = false;           /* the per-wave LDS allows 20 waves per CU */
+   int wide_waves = 4;             /* 3 where the LDS caps residency at 12 waves per CU */
    bool j45_fits = false;          /* the LDS layout holds the lockstep 4+5-cluster tile */
    struct Timed { int P; hipEvent_t a, b; };
    std::vector<Timed> timed;
@@ -173,8 +191,8 @@ struct kinc_sim
  * while a census is on and one exists for P, else the product build. */
 static const void* pair_kernel(const kinc_sim* h, int P)
 {
-   if ( h->base.census && P == 7 && h->kmax == 5 && !h->w5 ) return kinc_pair_kernel_census_k5_p7();
-   return kinc_select_pair_kernel(P, h->kmax, h->w5);
+   if ( h->base.census && P == 7 && h->kmax == 5 && h->waves == 4 ) return kinc_pair_kernel_census_k5_p7();
+   return kinc_select_pair_kernel(P, h->kmax, h->waves);
 }
 
 static int fail(kinc_sim* h, int code, const std::string& msg)
@@ -348,7 +366,11 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       a.lds_tile = a.lds_u;
       const size_t joint_bytes = std::max(sizeof(float) * 17 * (size_t)a.tile_stride,
                                           sizeof(float) * (size_t)a.tile_stride + (size_t)S);
-      size_t shared = std::max(std::max(tile_bytes, joint_bytes), sort_bytes);
+      /* the sort buffers serve outlier removal and Spearman only; K-means
+       * keeps its log-likelihood terms (N floats) there too */
+      const bool sorts = a.corr == 1 || a.pre_out || a.post_out;
+      size_t shared = std::max(std::max(tile_bytes, joint_bytes), sorts ? sort_bytes : (size_t)0);
+      if ( a.clus == 2 ) shared = std::max(shared, sizeof(float) * (size_t)S);
       if ( a.clus == 1 && a.max_k >= 5 && a.min_k <= 4 )
       {
          const size_t j45 = sizeof(float) * 29 * (size_t)a.tile_stride;
@@ -370,7 +392,12 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       h->j45_fits = a.joint45 != 0;
       h->lds_bytes = align16(off);
       h->w5_fits = h->lds_bytes * 20 <= (size_t)160 * 1024;
-      h->w5 = h->w5_fits;
+      /* waves per CU the per-wave LDS allows: at most 12, and the 3-wave
+       * builds reach the same residency without the 4-wave builds' 128-VGPR
+       * budget */
+      const size_t per_cu = ((size_t)160 * 1024) / h->lds_bytes;
+      h->wide_waves = per_cu <= 12 ? 3 : 4;
+      h->waves = h->w5_fits ? 5 : h->wide_waves;
    }
    else
    {
@@ -405,7 +432,8 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
          a.lds_tile = (int32_t)align16(sizeof(float) * 8 * 8);
          h->lds_bytes = align16((size_t)a.lds_tile + tile_bytes);
       }
-      h->w5 = false;
+      h->waves = 4;
+      h->wide_waves = 4;
    }
 }
 
@@ -898,15 +926,16 @@ extern "C" const char* kinc_sim_last_error(const kinc_sim* h)
 
 /*
  * The launch configurations a handle can run with and that give the same
- * results: the 5-waves-per-SIMD builds of the small register-slot instances
- * (on or off), how many auxiliary streams the persistent form's bins are
- * spread over, and whether the GMM fits with 4 and 5 clusters run in
- * lockstep (where the LDS layout holds their tile).  kinc_sim_autotune
- * times each on a sample of pairs and keeps the fastest.
+ * results: the waves per SIMD the register-slot bins' builds are made for
+ * (4; 5 for the small instances where the LDS leaves room for 20 waves per
+ * CU; 3 where it allows at most 12), how many auxiliary streams the
+ * persistent form's bins are spread over, and whether the GMM fits with 4
+ * and 5 clusters run in lockstep (where the LDS layout holds their tile).
+ * kinc_sim_autotune times each on a sample of pairs and keeps the fastest.
  */
-static int time_config(kinc_sim* h, int64_t start, int count, bool w5, int naux, bool j45, double* ms)
+static int time_config(kinc_sim* h, int64_t start, int count, int waves, int naux, bool j45, double* ms)
 {
-   h->w5 = w5;
+   h->waves = waves;
    h->naux = naux;
    h->base.joint45 = j45 ? 1 : 0;
    hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -934,6 +963,12 @@ static int time_config(kinc_sim* h, int64_t start, int count, bool w5, int naux,
    return rc;
 }
 
+/* the wave counts a handle's register-slot bins can run with */
+static bool waves_allowed(const kinc_sim* h, int waves)
+{
+   return waves == 4 || (waves == 5 && h->w5_fits) || (waves < 4 && waves == h->wide_waves);
+}
+
 extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc_sim_tuning* out)
 {
    if ( !h ) return KINC_E_INVALID;
@@ -945,27 +980,27 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
    int rc = ensure_staging(h);
    if ( rc ) return rc;
    HIP_TRY(h, hipSetDevice(h->device));
-   const bool saved_w5 = h->w5;
+   const int saved_waves = h->waves;
    const int saved_naux = h->naux;
    const bool saved_j45 = h->base.joint45 != 0;
-   /* candidates; the 5-wave builds and the lockstep fits only where the LDS
+   /* candidates; the other builds and the lockstep fits only where the LDS
     * leaves room for them */
-   const bool flags[2] = {false, true};
+   const int wavess[3] = {3, 4, 5};
    const int nauxs[3] = {1, 2, 4};
-   bool best_w5 = saved_w5, best_j45 = saved_j45;
-   int best_naux = saved_naux;
+   int best_waves = saved_waves, best_naux = saved_naux;
+   bool best_j45 = saved_j45;
    double best_ms = 1e30;
    int tried = 0;
    std::vector<int8_t> K0((size_t)count), K1((size_t)count);
    std::vector<float> C0((size_t)count * (size_t)h->p.max_clusters), C1(C0.size());
-   for ( int c = 0; c < 12 && rc == KINC_OK; ++c )
+   for ( int c = 0; c < 18 && rc == KINC_OK; ++c )
    {
-      const bool w5 = flags[c / 6];
-      const bool j45 = flags[(c / 3) & 1];
+      const int waves = wavess[c / 6];
+      const bool j45 = ((c / 3) & 1) != 0;
       const int naux = nauxs[c % 3];
-      if ( (w5 && !h->w5_fits) || (j45 && !h->j45_fits) ) continue;
+      if ( !waves_allowed(h, waves) || (j45 && !h->j45_fits) ) continue;
       double ms = 0.0;
-      rc = time_config(h, start, (int)count, w5, naux, j45, &ms);
+      rc = time_config(h, start, (int)count, waves, naux, j45, &ms);
       if ( rc ) break;
       /* every configuration must give the same bits */
       std::vector<int8_t>& K = tried == 0 ? K0 : K1;
@@ -986,18 +1021,18 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
       if ( ms < best_ms )
       {
          best_ms = ms;
-         best_w5 = w5;
+         best_waves = waves;
          best_naux = naux;
          best_j45 = j45;
       }
    }
-   h->w5 = rc ? saved_w5 : best_w5;
+   h->waves = rc ? saved_waves : best_waves;
    h->naux = rc ? saved_naux : best_naux;
    h->base.joint45 = (rc ? saved_j45 : best_j45) ? 1 : 0;
    if ( rc ) return rc;
    if ( out )
    {
-      out->w5 = h->w5 ? 1 : 0;
+      out->pair_waves = h->waves;
       out->aux_streams = h->naux;
       out->joint45 = h->base.joint45;
       out->candidates = tried;
@@ -1023,7 +1058,7 @@ extern "C" int kinc_selftest_census(kinc_sim* h, uint64_t* d_counters)
 extern "C" int kinc_sim_get_tuning(const kinc_sim* h, kinc_sim_tuning* out)
 {
    if ( !h || !out ) return KINC_E_INVALID;
-   out->w5 = h->w5 ? 1 : 0;
+   out->pair_waves = h->waves;
    out->aux_streams = h->naux;
    out->joint45 = h->base.joint45;
    out->candidates = 0;
@@ -1035,9 +1070,13 @@ extern "C" int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t)
 {
    if ( !h || !t ) return KINC_E_INVALID;
    if ( t->aux_streams < 1 || t->aux_streams > kinc_sim::kAux ) return fail_c(h, KINC_E_INVALID, "aux_streams must lie in [1, 4]");
-   if ( t->w5 && !h->w5_fits ) return fail_c(h, KINC_E_UNSUPPORTED, "the per-wave LDS leaves no room for 5 waves per SIMD");
+   if ( t->pair_waves < 3 || t->pair_waves > 5 ) return fail_c(h, KINC_E_INVALID, "pair_waves must be 3, 4 or 5");
+   if ( !waves_allowed(h, t->pair_waves) )
+   {
+      return fail_c(h, KINC_E_UNSUPPORTED, "the per-wave LDS makes no use of builds for that many waves per SIMD");
+   }
    if ( t->joint45 && !h->j45_fits ) return fail_c(h, KINC_E_UNSUPPORTED, "the LDS layout holds no tile for the lockstep 4- and 5-cluster fits");
-   h->w5 = t->w5 != 0;
+   h->waves = t->pair_waves;
    h->naux = t->aux_streams;
    h->base.joint45 = t->joint45 ? 1 : 0;
    return KINC_OK;

@@ -2184,13 +2184,14 @@ __device__ __forceinline__ FitOut fit_dispatch(const SimArgs& a, const Lds& L, i
    return f;
 }
 
-/* The fits with 4 and 5 clusters run in lockstep (a.joint45) on register-
- * slot instances of at most 7 slots built for 4 waves per SIMD: their nine
- * components' responsibilities take 63 of the 128 VGPRs. */
+/* The fits with 4 and 5 clusters run in lockstep (a.joint45) on the
+ * register-slot instances whose VGPR budget holds their nine components'
+ * responsibilities (9 P registers) besides X and log p(x) (4 P): at most 7
+ * slots in the 128 VGPRs of 4 waves per SIMD, 10 in the 168 of 3. */
 template <int P>
 __device__ __forceinline__ constexpr bool joint45_built()
 {
-   return P > 0 && P <= 7 && KINC_PAIR_WAVES == 4;
+   return P > 0 && ((KINC_PAIR_WAVES == 4 && P <= 7) || (KINC_PAIR_WAVES == 3 && P <= 10));
 }
 
 template <int P, int KMAX>

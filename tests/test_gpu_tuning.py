@@ -1,8 +1,9 @@
 """The launch-configuration autotuner (kinc_sim_autotune): every
-configuration a handle can take (5-wave builds of the small bins on or off,
-1, 2 or 4 auxiliary streams for the bins, the GMM fits with 4 and 5 clusters
-in lockstep or one after the other) gives the same bits, equal to the
-oracle's, and the tuner keeps one of them."""
+configuration a handle can take (the bins' kernel builds for 4 waves per
+SIMD or for 5, 3 or 2 where the LDS makes them useful, 1, 2 or 4 auxiliary
+streams for the bins, the GMM fits with 4 and 5 clusters in lockstep or one
+after the other) gives the same bits, equal to the oracle's, and the tuner
+keeps one of them."""
 import numpy as np
 import pytest
 
@@ -28,12 +29,14 @@ def test_every_configuration_gives_the_same_bits(S, flags):
         assert t["candidates"] == 6          # 256 and 300 samples leave room for 5-wave builds
         assert t["aux_streams"] in (1, 2, 4) and t["sample_ms"] > 0
         assert_block_equal(eng.execute(0, 3000), want, S, "tuned")
-        for w5 in (False, True):
+        for waves in (4, 5):
             for naux in (1, 2, 4):
-                eng.set_tuning(w5, naux)
-                assert_block_equal(eng.execute(0, 3000), want, S, "w5=%s aux=%d" % (w5, naux))
+                eng.set_tuning(waves, naux)
+                assert_block_equal(eng.execute(0, 3000), want, S, "waves=%d aux=%d" % (waves, naux))
         with pytest.raises(_native.KincError):
-            eng.set_tuning(False, 5)
+            eng.set_tuning(4, 5)
+        with pytest.raises(_native.KincError):
+            eng.set_tuning(3, 4)             # the LDS allows 20 waves per CU here
     finally:
         eng.close()
     assert_block_equal(want, oracle.execute_block(p, x, 0, 3000), S, "oracle")
@@ -44,10 +47,11 @@ def test_five_wave_builds_refused_where_lds_leaves_no_room():
     eng = _native.Engine(x, _native.make_params(clusmethod="gmm", corrmethod="spearman"))
     try:
         t = eng.autotune(0, 150)
-        # 900 samples: no 5-wave builds; the lockstep fits on or off
-        assert t["w5"] is False and t["candidates"] == 6
+        # 900 samples: no 5-wave builds; 3-wave builds or 4-wave ones, the
+        # lockstep fits on or off
+        assert t["pair_waves"] in (3, 4) and t["candidates"] == 12
         with pytest.raises(_native.KincError):
-            eng.set_tuning(True, 4)
+            eng.set_tuning(5, 4)
     finally:
         eng.close()
 
@@ -73,10 +77,10 @@ def test_lockstep_four_and_five_cluster_fits(S, nan, flags):
     try:
         assert eng.tuning()["joint45"] is True
         got = eng.execute(0, 3000)
-        eng.set_tuning(False, 4, joint45=False)
+        eng.set_tuning(4, 4, joint45=False)
         assert eng.tuning()["joint45"] is False
         assert_block_equal(eng.execute(0, 3000), got, S, "joint45 off")
-        eng.set_tuning(False, 2, joint45=True)
+        eng.set_tuning(4, 2, joint45=True)
         assert_block_equal(eng.execute(0, 3000), got, S, "joint45 on again")
     finally:
         eng.close()
@@ -89,6 +93,36 @@ def test_lockstep_fits_refused_where_lds_leaves_no_room():
     try:
         assert eng.tuning()["joint45"] is False
         with pytest.raises(_native.KincError):
-            eng.set_tuning(False, 4, joint45=True)
+            eng.set_tuning(4, 4, joint45=True)
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("S,flags", [
+    # per-wave LDS of about 17.5 KB: 9 waves per CU
+    (900, dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True)),
+    # about 18.7 KB: 8 waves per CU
+    (1024, dict(clusmethod="gmm", corrmethod="spearman", crit="ICL")),
+    # K-means + Pearson without outliers (no sort buffers): 10 waves per CU
+    (1000, dict(clusmethod="kmeans", corrmethod="pearson", crit="BIC")),
+])
+def test_wide_register_builds(S, flags):
+    """Where the LDS caps residency at 12 waves per CU the handle runs the
+    3-wave builds (and the lockstep fits on up to 10 slots); they give the
+    bits of the 4-wave builds and of the oracle."""
+    x = gemgen.modules(40, S, seed=35, nan_rate=0.2, nan_beta=True)
+    p = _native.make_params(**flags)
+    eng = _native.Engine(x, p)
+    n = 500
+    try:
+        assert eng.tuning()["pair_waves"] == 3
+        got = eng.execute(0, n)
+        for j45 in ((False, True) if flags["clusmethod"] == "gmm" else (False,)):
+            for w in (3, 4):
+                eng.set_tuning(w, 4, joint45=j45)
+                assert_block_equal(eng.execute(0, n), got, S, "waves=%d joint45=%s" % (w, j45))
+        with pytest.raises(_native.KincError):
+            eng.set_tuning(5, 4)
+    finally:
+        eng.close()
+    assert_block_equal(got, oracle.execute_block(p, x, 0, n), S, "oracle")
