@@ -345,12 +345,12 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
    {
       /* everything in LDS: labels, work labels, component parameters (nine
        * sets: the fits with 4 and 5 clusters run together), then the compact
-       * X and after it the region the chain tile shares with the sort
-       * buffers.  The GMM fits with 2 and 3 clusters run together on 17
-       * chain rows of the shared region, and the second fit's labels wait
-       * past the tile's first row; the fits with 4 and 5 clusters (29 rows)
-       * lay their tile over X and the shared region, which therefore follow
-       * each other. */
+       * X and after it the shared region that the single fits' chain tile,
+       * the sort buffers and K-means's log-likelihood terms take in turn.
+       * The GMM fits with 2 and 3 clusters (17 chain rows) and those with 4
+       * and 5 (29 rows) run in lockstep with their tile over X and the
+       * shared region, which therefore follow each other; the second fit's
+       * labels wait in the shared region past its first row. */
       size_t off = 0;
       a.scratch = nullptr;
       a.lds_lab = (int32_t)off;
@@ -364,16 +364,27 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       off += x_bytes;
       a.lds_u = (int32_t)off;
       a.lds_tile = a.lds_u;
-      const size_t joint_bytes = std::max(sizeof(float) * 17 * (size_t)a.tile_stride,
-                                          sizeof(float) * (size_t)a.tile_stride + (size_t)S);
-      /* the sort buffers serve outlier removal and Spearman only; K-means
-       * keeps its log-likelihood terms (N floats) there too */
+      const size_t row = sizeof(float) * (size_t)a.tile_stride;
+      /* the sort buffers serve outlier removal and Spearman only */
       const bool sorts = a.corr == 1 || a.pre_out || a.post_out;
-      size_t shared = std::max(std::max(tile_bytes, joint_bytes), sorts ? sort_bytes : (size_t)0);
-      if ( a.clus == 2 ) shared = std::max(shared, sizeof(float) * (size_t)S);
+      size_t shared = std::max(row, sorts ? sort_bytes : (size_t)0);
+      if ( a.clus == 1 )
+      {
+         /* a single fit's 1+3K rows; the joint fits' second labels after the
+          * entropy chain's row; the 17-row tile over X and this region */
+         shared = std::max(shared, tile_bytes);
+         shared = std::max(shared, row + (size_t)S);
+         if ( a.max_k >= 3 && a.min_k <= 2 && 17 * row > x_bytes ) shared = std::max(shared, 17 * row - x_bytes);
+      }
+      else if ( a.clus == 2 )
+      {
+         /* the mean chains' 2K rows, the log-likelihood terms (N floats) */
+         shared = std::max(shared, 2 * (size_t)h->kmax * row);
+         shared = std::max(shared, sizeof(float) * (size_t)S);
+      }
       if ( a.clus == 1 && a.max_k >= 5 && a.min_k <= 4 )
       {
-         const size_t j45 = sizeof(float) * 29 * (size_t)a.tile_stride;
+         const size_t j45 = 29 * row;
          const size_t need = j45 > x_bytes ? j45 - x_bytes : 0;
          const size_t cap = (size_t)160 * 1024;
          if ( need <= shared )

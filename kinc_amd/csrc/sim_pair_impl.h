@@ -1129,15 +1129,14 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
  * of the first fit is parameter set k; component k of the second is set
  * KA + k.  Labels of the first fit go to L.w, of the second to wb.
  *
- * The chain tile of up to 17 rows (KA + KB <= 5) is the shared tile region.
- * A larger pair of fits (4 and 5 clusters: 29 pass-A rows) lays its tile
- * over the compact X and the shared region after it, which the host places
- * next to each other: X is in registers for the whole fit and the tile is
- * only written after the initial draws have read X, and X[0, N) is written
- * back from the registers before the fit returns (X[N, S) is rewritten by
- * every later reader that needs it, fill_stale).  wb and the entropy chain's
- * row lie in the shared region past X, which the tile no longer uses once
- * the EM loop has ended.
+ * The chain tile (17 pass-A rows for the fits with 2 and 3 clusters, 29 for
+ * those with 4 and 5) lies over the compact X and the shared region after
+ * it, which the host places next to each other: X is in registers for the
+ * whole fit and the tile is only written after the initial draws have read
+ * X, and X[0, N) is written back from the registers before the fit returns
+ * (X[N, S) is rewritten by every later reader that needs it, fill_stale).
+ * wb and the entropy chain's row lie in the shared region past X, which the
+ * tile no longer uses once the EM loop has ended.
  */
 template <int P, int KT>
 struct JointStore
@@ -1314,8 +1313,8 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
    constexpr int NCA = RB + 1 + 3 * KB;    /* pass-A chains */
    const int lane = lane_id();
    const int nslots = (N + WAVE - 1) / WAVE;
-   /* more than 17 rows: the tile over X and the shared region after it */
-   float* tile = (NCA > 17) ? (float*)L.X : L.tile;
+   /* the tile over X and the shared region after it */
+   float* tile = (float*)L.X;
    float* prm = L.prm;
    CENSUS_T(ct);
    CENSUS_COUNT(KINC_CENSUS_FITS, 2);
@@ -1552,17 +1551,14 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
    fb.E = 0.0f;
    if ( !failA ) fa.E = joint_labels<P, KT, 0, 0, KA>(L, N, va, prm, convA, want_entropy, L.w);
    if ( !failB ) fb.E = joint_labels<P, KT, KA, 1, KB>(L, N, vb, prm, convB, want_entropy, wb);
-   if constexpr ( NCA > 17 )
-   {
-      /* the tile covered X: write the pair's points back */
+   /* the tile covered X: write the pair's points back */
 #pragma unroll
-      for ( int m = 0; m < P; ++m )
-      {
-         const int i = lane + m * WAVE;
-         if ( m < nslots && i < N ) L.X[i] = make_float2(js.x0[m], js.x1[m]);
-      }
-      wave_sync(false);
+   for ( int m = 0; m < P; ++m )
+   {
+      const int i = lane + m * WAVE;
+      if ( m < nslots && i < N ) L.X[i] = make_float2(js.x0[m], js.x1[m]);
    }
+   wave_sync(false);
    CENSUS_LAP(KINC_CENSUS_LABELS, ct);
 }
 
