@@ -187,17 +187,19 @@ int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launches,
  *                made for: 4; 5 (the instances of at most 6 slots, only when
  *                the per-wave LDS leaves room for 20 waves per CU); 3 (the
  *                instances of 5 slots or more, only when the per-wave LDS
- *                allows at most 12 waves per CU: the same residency with 168
- *                VGPRs per wave instead of 128).  A handle starts with 5
- *                where it fits, else 3 where the LDS caps residency at 12,
- *                else 4
+ *                allows fewer than 16 waves per CU: 168 VGPRs per wave
+ *                instead of 128, at the same residency where the LDS allows
+ *                at most 12).  A handle starts with 5 where it fits, else 3
+ *                where it is allowed, else 4
  *   aux_streams  streams the persistent form's bins run on concurrently (1-4)
  *   joint45      1: the GMM fits with 4 and 5 clusters run in lockstep on the
  *                register-slot instances whose VGPR budget holds them (at
  *                most 7 slots at 4 waves per SIMD, 10 at 3), sharing
- *                their chain-sum passes (only when the LDS layout holds their
- *                29-row tile: about 410 samples or more); 0: one after the
- *                other.  A handle starts with 1 where it fits.
+ *                their chain-sum passes; their 29-row tile lies over the
+ *                compact X and the shared LDS region, enlarging that region
+ *                where the two are smaller (offered while 8 waves per CU
+ *                stay resident); 0: one after the other.  A handle starts
+ *                with 1 where it is offered.
  * kinc_sim_autotune runs the sample of pairs [start, start+count) (at most
  * one work block) with every candidate configuration, three times each,
  * checks that all of them give identical K and correlations, keeps the

@@ -169,19 +169,18 @@ struct kinc_sim
    hipEvent_t binned = nullptr;
    hipEvent_t drained[kAux] = {};
    SimArgs base{};
-   size_t lds_bytes = 0;
+   size_t lds_bytes = 0;           /* per-wave LDS of the current configuration */
+   size_t lds_small = 0;           /* per-wave LDS without room for the lockstep 4+5-cluster tile */
+   size_t lds_j45 = 0;             /* per-wave LDS with that room, or 0 when the fits cannot run so */
    char err[256] = {};
    /* kernel timing */
    bool profiling = false;
    bool capturing = false;         /* between kinc_sim_graph_begin and _end */
    /* waves per SIMD of the pair-kernel builds the register-slot bins run:
     * 4, or 5 for the small instances where the per-wave LDS leaves room for
-    * 20 waves per CU, or 3 where it allows at most 12 (the same residency
-    * with 168 VGPRs per wave) */
+    * 20 waves per CU, or 3 (168 VGPRs per wave) where it allows fewer than
+    * 16 */
    int waves = 4;
-   bool w5_fits = false;           /* the per-wave LDS allows 20 waves per CU */
-   int wide_waves = 4;             /* 3 where the LDS caps residency at 12 waves per CU */
-   bool j45_fits = false;          /* the LDS layout holds the lockstep 4+5-cluster tile */
    struct Timed { int P; hipEvent_t a, b; };
    std::vector<Timed> timed;
    std::vector<hipEvent_t> event_pool;
@@ -298,6 +297,44 @@ static int validate(const kinc_sim_params* p, int G, int S, const char** why)
    return KINC_OK;
 }
 
+static const size_t kLdsPerCu = (size_t)160 * 1024;
+
+/* per-wave LDS of a handle with or without room for the lockstep 4- and
+ * 5-cluster tile */
+static size_t lds_for(const kinc_sim* h, bool j45)
+{
+   return j45 ? h->lds_j45 : h->lds_small;
+}
+
+/* the 5-wave builds of the small register-slot instances: where the LDS
+ * leaves room for 20 waves per CU */
+static bool w5_ok(const kinc_sim* h, bool j45)
+{
+   return h->classes[0] != 0 && lds_for(h, j45) * 20 <= kLdsPerCu;
+}
+
+/* the 3-wave builds (168 VGPRs) of the instances of 5 slots or more: where
+ * the handle has such bins and the LDS allows fewer than 16 waves per CU */
+static bool w3_ok(const kinc_sim* h, bool j45)
+{
+   return h->classes[0] != 0 && h->S > 4 * 64 && kLdsPerCu / lds_for(h, j45) < 16;
+}
+
+/* the wave counts a handle's register-slot bins can run with */
+static bool waves_allowed(const kinc_sim* h, int waves, bool j45)
+{
+   return waves == 4 || (waves == 5 && w5_ok(h, j45)) || (waves == 3 && w3_ok(h, j45));
+}
+
+/* set a launch configuration (checked by the caller) */
+static void apply_config(kinc_sim* h, int waves, int naux, bool j45)
+{
+   h->waves = waves;
+   h->naux = naux;
+   h->base.joint45 = j45 ? 1 : 0;
+   h->lds_bytes = lds_for(h, j45);
+}
+
 /*
  * The parameters, bins and working-set carve-up of a handle for an S-sample
  * matrix: everything but the device buffers.  Allocates nothing (the
@@ -345,12 +382,13 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
    {
       /* everything in LDS: labels, work labels, component parameters (nine
        * sets: the fits with 4 and 5 clusters run together), then the compact
-       * X and after it the shared region that the single fits' chain tile,
-       * the sort buffers and K-means's log-likelihood terms take in turn.
-       * The GMM fits with 2 and 3 clusters (17 chain rows) and those with 4
-       * and 5 (29 rows) run in lockstep with their tile over X and the
-       * shared region, which therefore follow each other; the second fit's
-       * labels wait in the shared region past its first row. */
+       * X and after it the shared region that the sort buffers and K-means's
+       * chain tile and log-likelihood terms take in turn.  A GMM fit holds X
+       * in registers and lays its chain tile over X and the shared region,
+       * which therefore follow each other: 1+3K rows for a single fit, 17
+       * for the fits with 2 and 3 clusters in lockstep and 29 for those with
+       * 4 and 5; the second lockstep fit's labels wait in the shared region
+       * past its first row. */
       size_t off = 0;
       a.scratch = nullptr;
       a.lds_lab = (int32_t)off;
@@ -370,11 +408,13 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       size_t shared = std::max(row, sorts ? sort_bytes : (size_t)0);
       if ( a.clus == 1 )
       {
-         /* a single fit's 1+3K rows; the joint fits' second labels after the
-          * entropy chain's row; the 17-row tile over X and this region */
-         shared = std::max(shared, tile_bytes);
+         /* the joint fits' second labels after the entropy chain's row; a
+          * single fit's 1+3K rows and the lockstep fits' 17 over X and this
+          * region */
          shared = std::max(shared, row + (size_t)S);
-         if ( a.max_k >= 3 && a.min_k <= 2 && 17 * row > x_bytes ) shared = std::max(shared, 17 * row - x_bytes);
+         size_t over_x = tile_bytes;
+         if ( a.max_k >= 3 && a.min_k <= 2 ) over_x = std::max(over_x, 17 * row);
+         if ( over_x > x_bytes ) shared = std::max(shared, over_x - x_bytes);
       }
       else if ( a.clus == 2 )
       {
@@ -382,33 +422,23 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
          shared = std::max(shared, 2 * (size_t)h->kmax * row);
          shared = std::max(shared, sizeof(float) * (size_t)S);
       }
+      h->lds_small = align16(off + shared);
+      h->lds_j45 = 0;
       if ( a.clus == 1 && a.max_k >= 5 && a.min_k <= 4 )
       {
-         const size_t j45 = 29 * row;
-         const size_t need = j45 > x_bytes ? j45 - x_bytes : 0;
-         const size_t cap = (size_t)160 * 1024;
-         if ( need <= shared )
-         {
-            a.joint45 = 1;
-         }
-         else if ( align16(off + shared) * 20 > cap && align16(off + need) * 16 <= cap )
-         {
-            /* a larger shared region costs no residency here: the 5-wave
-             * builds do not fit either way and 16 waves per CU still do */
-            shared = need;
-            a.joint45 = 1;
-         }
+         /* the lockstep 4+5-cluster tile: 29 rows over X and the shared
+          * region, offered while at least 8 waves per CU stay resident */
+         const size_t need = 29 * row > x_bytes ? 29 * row - x_bytes : 0;
+         const size_t lds = align16(off + std::max(shared, need));
+         if ( lds * 8 <= kLdsPerCu ) h->lds_j45 = lds;
       }
-      off += shared;
-      h->j45_fits = a.joint45 != 0;
-      h->lds_bytes = align16(off);
-      h->w5_fits = h->lds_bytes * 20 <= (size_t)160 * 1024;
-      /* waves per CU the per-wave LDS allows: at most 12, and the 3-wave
-       * builds reach the same residency without the 4-wave builds' 128-VGPR
-       * budget */
-      const size_t per_cu = ((size_t)160 * 1024) / h->lds_bytes;
-      h->wide_waves = per_cu <= 12 ? 3 : 4;
-      h->waves = h->w5_fits ? 5 : h->wide_waves;
+      /* the lockstep fits where they can run; 5-wave builds where the LDS
+       * then leaves room for 20 waves per CU, 3-wave ones where it allows
+       * fewer than 16 (their 168 VGPRs beat the 4-wave builds' spills at
+       * 12-15 waves per CU) */
+      const bool j45 = h->lds_j45 != 0;
+      const int waves = w5_ok(h, j45) ? 5 : (w3_ok(h, j45) ? 3 : 4);
+      apply_config(h, waves, h->naux, j45);
    }
    else
    {
@@ -443,8 +473,9 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
          a.lds_tile = (int32_t)align16(sizeof(float) * 8 * 8);
          h->lds_bytes = align16((size_t)a.lds_tile + tile_bytes);
       }
+      h->lds_small = h->lds_bytes;
+      h->lds_j45 = 0;
       h->waves = 4;
-      h->wide_waves = 4;
    }
 }
 
@@ -939,16 +970,14 @@ extern "C" const char* kinc_sim_last_error(const kinc_sim* h)
  * The launch configurations a handle can run with and that give the same
  * results: the waves per SIMD the register-slot bins' builds are made for
  * (4; 5 for the small instances where the LDS leaves room for 20 waves per
- * CU; 3 where it allows at most 12), how many auxiliary streams the
+ * CU; 3 where it allows fewer than 16), how many auxiliary streams the
  * persistent form's bins are spread over, and whether the GMM fits with 4
  * and 5 clusters run in lockstep (where the LDS layout holds their tile).
  * kinc_sim_autotune times each on a sample of pairs and keeps the fastest.
  */
 static int time_config(kinc_sim* h, int64_t start, int count, int waves, int naux, bool j45, double* ms)
 {
-   h->waves = waves;
-   h->naux = naux;
-   h->base.joint45 = j45 ? 1 : 0;
+   apply_config(h, waves, naux, j45);
    hipEvent_t e0 = nullptr, e1 = nullptr;
    HIP_TRY(h, hipEventCreate(&e0));
    int rc = KINC_OK;
@@ -972,12 +1001,6 @@ static int time_config(kinc_sim* h, int64_t start, int count, int waves, int nau
    if ( e1 ) (void)hipEventDestroy(e1);
    *ms = best;
    return rc;
-}
-
-/* the wave counts a handle's register-slot bins can run with */
-static bool waves_allowed(const kinc_sim* h, int waves)
-{
-   return waves == 4 || (waves == 5 && h->w5_fits) || (waves < 4 && waves == h->wide_waves);
 }
 
 extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc_sim_tuning* out)
@@ -1009,7 +1032,7 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
       const int waves = wavess[c / 6];
       const bool j45 = ((c / 3) & 1) != 0;
       const int naux = nauxs[c % 3];
-      if ( !waves_allowed(h, waves) || (j45 && !h->j45_fits) ) continue;
+      if ( (j45 && h->lds_j45 == 0) || !waves_allowed(h, waves, j45) ) continue;
       double ms = 0.0;
       rc = time_config(h, start, (int)count, waves, naux, j45, &ms);
       if ( rc ) break;
@@ -1037,10 +1060,12 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
          best_j45 = j45;
       }
    }
-   h->waves = rc ? saved_waves : best_waves;
-   h->naux = rc ? saved_naux : best_naux;
-   h->base.joint45 = (rc ? saved_j45 : best_j45) ? 1 : 0;
-   if ( rc ) return rc;
+   if ( rc )
+   {
+      apply_config(h, saved_waves, saved_naux, saved_j45);
+      return rc;
+   }
+   apply_config(h, best_waves, best_naux, best_j45);
    if ( out )
    {
       out->pair_waves = h->waves;
@@ -1082,14 +1107,12 @@ extern "C" int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t)
    if ( !h || !t ) return KINC_E_INVALID;
    if ( t->aux_streams < 1 || t->aux_streams > kinc_sim::kAux ) return fail_c(h, KINC_E_INVALID, "aux_streams must lie in [1, 4]");
    if ( t->pair_waves < 3 || t->pair_waves > 5 ) return fail_c(h, KINC_E_INVALID, "pair_waves must be 3, 4 or 5");
-   if ( !waves_allowed(h, t->pair_waves) )
+   if ( t->joint45 && h->lds_j45 == 0 ) return fail_c(h, KINC_E_UNSUPPORTED, "the LDS layout holds no tile for the lockstep 4- and 5-cluster fits");
+   if ( !waves_allowed(h, t->pair_waves, t->joint45 != 0) )
    {
       return fail_c(h, KINC_E_UNSUPPORTED, "the per-wave LDS makes no use of builds for that many waves per SIMD");
    }
-   if ( t->joint45 && !h->j45_fits ) return fail_c(h, KINC_E_UNSUPPORTED, "the LDS layout holds no tile for the lockstep 4- and 5-cluster fits");
-   h->waves = t->pair_waves;
-   h->naux = t->aux_streams;
-   h->base.joint45 = t->joint45 ? 1 : 0;
+   apply_config(h, t->pair_waves, t->aux_streams, t->joint45 != 0);
    return KINC_OK;
 }
 

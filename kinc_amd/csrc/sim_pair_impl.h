@@ -794,7 +794,24 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
    const int lane = lane_id();
    constexpr int STR = KINC_TILE_STRIDE;
    const int nslots = (N + WAVE - 1) / WAVE;
-   float* tile = L.tile;
+   /* a register store holds X for the whole fit, so the chain tile lies over
+    * X and the shared region after it (written only after the initial draws
+    * have read X; X[0, N) is written back before the fit returns, as in
+    * gmm_fit_joint); the memory store's X is in its scratch slice and the
+    * tile in LDS */
+   float* tile = (ST::PMAX > 0) ? (float*)L.X : L.tile;
+   auto restore_x = [&]() {
+      if constexpr ( ST::PMAX > 0 )
+      {
+#pragma unroll
+         for ( int m = 0; m < ST::PMAX; ++m )
+         {
+            const int i = lane + m * WAVE;
+            if ( m < nslots && i < N ) L.X[i] = make_float2(st.X0(m), st.X1(m));
+         }
+         wave_sync(false);
+      }
+   };
    float* prm = L.prm;
    FitOut out;
    out.ok = true;
@@ -1047,6 +1064,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
    if ( failed )
    {
       wave_sync(L.lgbuf != nullptr);
+      restore_x();
       out.ok = false;
       return out;
    }
@@ -1113,6 +1131,7 @@ __device__ __forceinline__ FitOut gmm_fit(const SimArgs& a, const Lds& L, int N,
       out.E = bcast(accE, 0);
       CENSUS_LAP(KINC_CENSUS_ENTROPY, ct);
    }
+   restore_x();
    return out;
 }
 

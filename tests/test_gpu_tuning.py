@@ -14,11 +14,14 @@ from parity_util import assert_block_equal
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("S,flags", [
-    (256, dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True)),
-    (300, dict(clusmethod="kmeans", corrmethod="pearson", crit="ICL")),
+@pytest.mark.parametrize("S,flags,configs", [
+    # 256 samples: the 5-wave builds (small layout) or the lockstep fits (their
+    # tile enlarges the layout past room for 20 waves per CU) on 4-wave builds
+    (256, dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True),
+     [(4, False), (4, True), (5, False)]),
+    (300, dict(clusmethod="kmeans", corrmethod="pearson", crit="ICL"), [(4, False), (5, False)]),
 ])
-def test_every_configuration_gives_the_same_bits(S, flags):
+def test_every_configuration_gives_the_same_bits(S, flags, configs):
     x = gemgen.modules(140, S, seed=31, nan_rate=0.15)
     p = _native.make_params(**flags)
     eng = _native.Engine(x, p)
@@ -26,17 +29,22 @@ def test_every_configuration_gives_the_same_bits(S, flags):
         want = eng.execute(0, 3000)
         t = eng.autotune(500, 2000)
         print("tuning", t, flush=True)
-        assert t["candidates"] == 6          # 256 and 300 samples leave room for 5-wave builds
+        assert t["candidates"] == 3 * len(configs)
+        assert (t["pair_waves"], t["joint45"]) in configs
         assert t["aux_streams"] in (1, 2, 4) and t["sample_ms"] > 0
         assert_block_equal(eng.execute(0, 3000), want, S, "tuned")
-        for waves in (4, 5):
+        for waves, j45 in configs:
             for naux in (1, 2, 4):
-                eng.set_tuning(waves, naux)
-                assert_block_equal(eng.execute(0, 3000), want, S, "waves=%d aux=%d" % (waves, naux))
+                eng.set_tuning(waves, naux, joint45=j45)
+                assert_block_equal(eng.execute(0, 3000), want, S,
+                                   "waves=%d joint45=%s aux=%d" % (waves, j45, naux))
         with pytest.raises(_native.KincError):
-            eng.set_tuning(4, 5)
+            eng.set_tuning(4, 5, joint45=False)
         with pytest.raises(_native.KincError):
-            eng.set_tuning(3, 4)             # the LDS allows 20 waves per CU here
+            eng.set_tuning(3, 4, joint45=False)  # no bins of 5 slots or more here
+        if (5, True) not in configs and (4, True) in configs:
+            with pytest.raises(_native.KincError):
+                eng.set_tuning(5, 4, joint45=True)
     finally:
         eng.close()
     assert_block_equal(want, oracle.execute_block(p, x, 0, 3000), S, "oracle")
@@ -87,9 +95,10 @@ def test_lockstep_four_and_five_cluster_fits(S, nan, flags):
     assert_block_equal(got, oracle.execute_block(p, x, 0, 3000), S, "oracle")
 
 
-def test_lockstep_fits_refused_where_lds_leaves_no_room():
-    x = gemgen.modules(20, 256, seed=34, nan_rate=0.1)
-    eng = _native.Engine(x, _native.make_params(clusmethod="gmm", corrmethod="spearman"))
+def test_lockstep_fits_refused_without_four_and_five_cluster_fits():
+    x = gemgen.modules(20, 480, seed=34, nan_rate=0.1)
+    eng = _native.Engine(x, _native.make_params(clusmethod="gmm", corrmethod="spearman",
+                                                maxclus=4))
     try:
         assert eng.tuning()["joint45"] is False
         with pytest.raises(_native.KincError):
