@@ -220,6 +220,30 @@ int kinc_sim_get_tuning(const kinc_sim* h, kinc_sim_tuning* out);
 int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t);
 
 /*
+ * The launch plan a handle would get for these parameters and this matrix
+ * shape, computed on the host without a device (no counterpart in the
+ * reference, whose OpenCL worker sizes nothing per shape; for callers that
+ * size work before creating handles, and for tests of the layout rules):
+ * the register-slot classes of the bins (0 = the memory-store instance), the
+ * per-wave LDS bytes without and with room for the lockstep 4- and 5-cluster
+ * tile (0: not offered), a bit mask of the pair_waves values kinc_sim_set_
+ * tuning accepts for some joint45 setting (bit w), and the configuration
+ * the handle starts with.  Returns 0 or a negative code as kinc_sim_create
+ * would for the same arguments (before touching a device).
+ */
+typedef struct kinc_sim_plan
+{
+   int32_t bins;
+   int32_t slots[12];
+   int32_t lds_small;
+   int32_t lds_joint45;
+   int32_t pair_waves_allowed;
+   kinc_sim_tuning tuning;
+} kinc_sim_plan;
+
+int kinc_sim_plan_get(const kinc_sim_params* params, int32_t G, int32_t S, kinc_sim_plan* out);
+
+/*
  * HIP graphs of work blocks (no counterpart in the reference, whose OpenCL
  * worker enqueues every kernel of every sub-batch itself).  Between
  * kinc_sim_graph_begin and kinc_sim_graph_end, kinc_sim_execute_device and

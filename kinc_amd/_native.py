@@ -75,6 +75,7 @@ SIGNATURES = [
     ("kinc_sim_autotune", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P]),
     ("kinc_sim_get_tuning", ctypes.c_int, [_P, _P]),
     ("kinc_sim_set_tuning", ctypes.c_int, [_P, _P]),
+    ("kinc_sim_plan_get", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P]),
     ("kinc_sim_graph_begin", ctypes.c_int, [_P]),
     ("kinc_sim_graph_end", ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     ("kinc_sim_graph_launch", ctypes.c_int, [_P, _P]),
@@ -146,6 +147,27 @@ class KincError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("kinc error %d: %s" % (code, msg))
         self.code = code
+
+
+class Plan(ctypes.Structure):
+    """Mirror of kinc_sim_plan (include/kinc_sim.h)."""
+    _fields_ = [("bins", ctypes.c_int32), ("slots", ctypes.c_int32 * 12),
+                ("lds_small", ctypes.c_int32), ("lds_joint45", ctypes.c_int32),
+                ("pair_waves_allowed", ctypes.c_int32), ("tuning", Tuning)]
+
+
+def plan(params, G, S):
+    """The launch plan a handle would get (kinc_sim_plan_get; no device):
+    bins' slot counts, per-wave LDS without / with the lockstep 4+5-cluster
+    tile, the accepted pair_waves values and the starting configuration."""
+    pl = Plan()
+    rc = lib().kinc_sim_plan_get(ctypes.byref(params), G, S, ctypes.byref(pl))
+    if rc != 0:
+        raise KincError(rc, lib().kinc_sim_last_error(None).decode())
+    return {"slots": [int(pl.slots[i]) for i in range(pl.bins)],
+            "lds_small": int(pl.lds_small), "lds_joint45": int(pl.lds_joint45),
+            "pair_waves_allowed": [w for w in range(32) if pl.pair_waves_allowed >> w & 1],
+            "tuning": pl.tuning.as_dict()}
 
 
 def pair_count(G):

@@ -300,7 +300,7 @@ static int validate(const kinc_sim_params* p, int G, int S, const char** why)
 static const size_t kLdsPerCu = (size_t)160 * 1024;
 
 /* per-wave LDS of a handle with or without room for the lockstep 4- and
- * 5-cluster tile */
+ * 5-cluster tile (0 with it when those fits are not offered) */
 static size_t lds_for(const kinc_sim* h, bool j45)
 {
    return j45 ? h->lds_j45 : h->lds_small;
@@ -310,14 +310,16 @@ static size_t lds_for(const kinc_sim* h, bool j45)
  * leaves room for 20 waves per CU */
 static bool w5_ok(const kinc_sim* h, bool j45)
 {
-   return h->classes[0] != 0 && lds_for(h, j45) * 20 <= kLdsPerCu;
+   const size_t lds = lds_for(h, j45);
+   return h->classes[0] != 0 && lds != 0 && lds * 20 <= kLdsPerCu;
 }
 
 /* the 3-wave builds (168 VGPRs) of the instances of 5 slots or more: where
  * the handle has such bins and the LDS allows fewer than 16 waves per CU */
 static bool w3_ok(const kinc_sim* h, bool j45)
 {
-   return h->classes[0] != 0 && h->S > 4 * 64 && kLdsPerCu / lds_for(h, j45) < 16;
+   const size_t lds = lds_for(h, j45);
+   return h->classes[0] != 0 && lds != 0 && h->S > 4 * 64 && kLdsPerCu / lds < 16;
 }
 
 /* the wave counts a handle's register-slot bins can run with */
@@ -477,6 +479,29 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       h->lds_j45 = 0;
       h->waves = 4;
    }
+}
+
+extern "C" int kinc_sim_plan_get(const kinc_sim_params* params, int32_t G, int32_t S,
+                                 kinc_sim_plan* out)
+{
+   if ( !params || !out ) return fail(nullptr, KINC_E_INVALID, "null argument");
+   const char* why = nullptr;
+   const int rc = validate(params, G, S, &why);
+   if ( rc != KINC_OK ) return fail_c(nullptr, rc, why);
+   /* the layout a handle would get (setup_layout touches no device) */
+   kinc_sim h;
+   setup_layout(&h, params, G, S);
+   std::memset(out, 0, sizeof(*out));
+   out->bins = h.nbins;
+   for ( int b = 0; b < h.nbins && b < (int)(sizeof(out->slots) / sizeof(out->slots[0])); ++b ) out->slots[b] = h.classes[b];
+   out->lds_small = (int32_t)h.lds_small;
+   out->lds_joint45 = (int32_t)h.lds_j45;
+   out->pair_waves_allowed = (1 << 4) | ((w5_ok(&h, false) || w5_ok(&h, true)) ? 1 << 5 : 0) |
+                             ((w3_ok(&h, false) || w3_ok(&h, true)) ? 1 << 3 : 0);
+   out->tuning.pair_waves = h.waves;
+   out->tuning.aux_streams = h.naux;
+   out->tuning.joint45 = h.base.joint45;
+   return KINC_OK;
 }
 
 extern "C" int kinc_sim_create(const kinc_sim_params* params, const float* emx,
