@@ -339,8 +339,8 @@ static void apply_config(kinc_sim* h, int waves, int naux, bool j45)
 
 /*
  * The parameters, bins and working-set carve-up of a handle for an S-sample
- * matrix: everything but the device buffers.  Allocates nothing (the
- * allocation-free entry point uses it too).
+ * matrix: everything but the device buffers.  Allocates nothing and touches
+ * no device (kinc_sim_plan_get runs it on a handle of its own).
  */
 static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int S)
 {
@@ -808,21 +808,17 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       int rc = launch(h, kinc_kernel_bin_pairs(), (unsigned)((cnt + 255) / 256), 256, args, 0, s);
       if ( rc ) return rc;
    }
-   /* Bins run concurrently: a bin with few pairs would otherwise leave most
-    * of the device idle for one pair's latency (milliseconds).  Without
-    * auxiliary streams (the allocation-free entry point) they run one after
-    * another on s. */
-   const bool fork = h->aux[0] != nullptr;
-   const int naux = fork ? std::max(1, std::min(h->naux, (int)kinc_sim::kAux)) : 1;
-   if ( fork )
-   {
-      HIP_TRY(h, hipEventRecord(h->binned, s));
-      for ( int i = 0; i < naux; ++i ) HIP_TRY(h, hipStreamWaitEvent(h->aux[i], h->binned, 0));
-   }
+   /* Bins run concurrently on the handle's auxiliary streams: a bin with few
+    * pairs would otherwise leave most of the device idle for one pair's
+    * latency (milliseconds).  They wait for the binning on s, and s waits
+    * for all of them. */
+   const int naux = std::max(1, std::min(h->naux, (int)kinc_sim::kAux));
+   HIP_TRY(h, hipEventRecord(h->binned, s));
+   for ( int i = 0; i < naux; ++i ) HIP_TRY(h, hipStreamWaitEvent(h->aux[i], h->binned, 0));
    for ( int b = h->nbins - 1; b >= 0; --b )
    {
       const int P = h->classes[b];
-      hipStream_t sb = fork ? h->aux[(h->nbins - 1 - b) % naux] : s;
+      hipStream_t sb = h->aux[(h->nbins - 1 - b) % naux];
       const void* fn = pair_kernel(h, P);
       if ( !fn ) return fail_c(h, KINC_E_UNSUPPORTED, "no kernel instance for this shape");
       int per_cu = 0;
@@ -838,7 +834,7 @@ static int run(kinc_sim* h, int64_t start, const int32_t* pairs, const int32_t* 
       int rc = launch_pairs(h, P, fn, (unsigned)grid, args, sb);
       if ( rc ) return rc;
    }
-   for ( int i = 0; fork && i < naux; ++i )
+   for ( int i = 0; i < naux; ++i )
    {
       HIP_TRY(h, hipEventRecord(h->drained[i], h->aux[i]));
       HIP_TRY(h, hipStreamWaitEvent(s, h->drained[i], 0));
