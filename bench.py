@@ -622,6 +622,14 @@ def rmt_leg(gemgen, with_cpu):
     return out
 
 
+# VALU instructions per operation of the probe modes that run compiled code:
+# one exponential, as the pair kernels compile it (kinc_expf_le0: one clamp,
+# multiply, v_rndne, two reduction FMAs, six polynomial FMAs, v_cvt, v_ldexp;
+# kinc_expf: a second clamp), counted in the probe loop's gfx950 assembly,
+# which holds exactly 128 of them and no other VALU instruction
+PROBE_VALU_PER_OP = {"exp_le0": 13, "exp": 14}
+
+
 def valu_calibration():
     """Issue rate of VALU instruction streams with 16 one-wave workgroups per
     CU (4 waves per SIMD, as the pair kernels run): wave64 instructions per
@@ -640,10 +648,10 @@ def valu_calibration():
     rows = []
     for mode, (chains, lanes, op) in enumerate(native.VALU_PROBE_MODES):
         ms = native.valu_probe(mode, iters, wgs, lds)
-        insts = 128.0 * iters * wgs
+        name = native.VALU_PROBE_OPS[op]
+        insts = 128.0 * iters * wgs * PROBE_VALU_PER_OP.get(name, 1)
         rate = insts / (ms * 1e-3)
         cpi = (cus * 4) * CLOCK_HZ / rate
-        name = native.VALU_PROBE_OPS[op]
         rows.append({"mode": mode, "op": name, "chains": chains, "lanes": lanes, "ms": ms,
                      "wave_insts": insts, "insts_per_s": rate,
                      "cycles_per_inst_per_simd": cpi})

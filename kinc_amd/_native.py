@@ -362,15 +362,18 @@ def census_dict(values):
 
 # (independent chains, active lanes, instruction) of each kinc_selftest_valu_probe mode
 VALU_PROBE_OPS = ["v_add_f32", "v_mul_f32", "v_fma_f32", "v_fmac_f32", "v_max_f32",
-                  "v_rndne_f32", "v_ldexp_f32", "v_cvt_i32_f32", "v_cndmask_b32", "v_fmaak_f32"]
+                  "v_rndne_f32", "v_ldexp_f32", "v_cvt_i32_f32", "v_cndmask_b32", "v_fmaak_f32",
+                  "exp_le0", "exp"]
 VALU_PROBE_MODES = [(8, 64, 0), (8, 16, 0), (8, 4, 0), (1, 64, 0), (1, 16, 0), (8, 32, 0),
                     (8, 64, 1), (8, 64, 2), (8, 64, 3), (8, 64, 4), (8, 64, 5), (8, 64, 6),
-                    (8, 64, 7), (8, 64, 8), (8, 64, 9), (1, 64, 2), (8, 16, 2)]
+                    (8, 64, 7), (8, 64, 8), (8, 64, 9), (1, 64, 2), (8, 16, 2), (8, 64, 10),
+                    (8, 64, 11)]
 
 
 def valu_probe(mode, iters, workgroups, lds_bytes):
-    """Duration in ms of one probe launch issuing 128 * iters v_add_f32 per
-    wave on the lanes and chains of `mode` (kinc_selftest_valu_probe)."""
+    """Duration in ms of one probe launch issuing 128 * iters operations per
+    wave (one instruction each, or one exponential for the exp modes) on the
+    lanes and chains of `mode` (kinc_selftest_valu_probe)."""
     ms = ctypes.c_float()
     rc = lib().kinc_selftest_valu_probe(mode, iters, workgroups, lds_bytes, ctypes.byref(ms))
     if rc != KINC_OK:

@@ -1161,9 +1161,18 @@ template <int P, int KT>
 struct JointStore
 {
    static constexpr int PMAX = P;
+   /* With nine components the log p(x) values do not fit the register
+    * budget beside the responsibilities and X: they live in the wave's
+    * private memory (the address escapes below, so the compiler keeps the
+    * array there), written by the E step and read by pass A one slot ahead
+    * of their use, so the load latency hides behind a slot's work.  Left to
+    * the register allocator they were spilled and reloaded right before the
+    * use, each reload a full memory round trip. */
+   static constexpr bool LPX_MEM = KT > 5;
    float x0[P], x1[P];
    float lg[KT][P];
-   float lpx[2][P];
+   float lpx[2][P];      /* fewer than nine components */
+   float* lpm;           /* nine: [2][P] in the fit's private array */
 
    __device__ __forceinline__ void init(const Lds& L, int N)
    {
@@ -1188,7 +1197,11 @@ struct JointView
    __device__ __forceinline__ float X0(int m) const { return js.x0[m]; }
    __device__ __forceinline__ float X1(int m) const { return js.x1[m]; }
    __device__ __forceinline__ float& LG(int k, int m) { return js.lg[OFF + k][m]; }
-   __device__ __forceinline__ float& LPX(int m) { return js.lpx[F][m]; }
+   __device__ __forceinline__ float& LPX(int m)
+   {
+      if constexpr ( JointStore<P, KT>::LPX_MEM ) return js.lpm[F * P + m];
+      else return js.lpx[F][m];
+   }
 };
 
 /* calcLabels and the entropy terms of one finished fit (as in gmm_fit) */
@@ -1253,14 +1266,15 @@ __device__ __forceinline__ float joint_labels(const Lds& L, int N, JointView<P, 
    return bcast(accE, 0);
 }
 
-/* pass-A terms of one fit into tile rows R.. (lp, K gamma-sum terms, 2K
- * mean terms) and the exponentiated responsibilities into LG */
+/* pass-A terms of one fit into tile rows R.. (lp = LPX(m), K gamma-sum
+ * terms, 2K mean terms) and the exponentiated responsibilities into LG */
 template <class V, int K>
-__device__ __forceinline__ void joint_terms_a(V& v, int m, float* tile, int R, const float* gmax)
+__device__ __forceinline__ void joint_terms_a(V& v, int m, float* tile, int R, const float* gmax,
+                                              float lp)
 {
    constexpr int STR = KINC_TILE_STRIDE;
    const int lane = lane_id();
-   tile[R * STR + lane] = v.LPX(m);
+   tile[R * STR + lane] = lp;
    const float xa = v.X0(m);
    const float xb = v.X1(m);
 #pragma unroll
@@ -1339,6 +1353,11 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
    CENSUS_COUNT(KINC_CENSUS_FITS, 2);
 
    JointStore<P, KT> js;
+   /* the private array of log p(x) (nine components): its address escapes
+    * here, so it stays in memory and apart from the register-held fields */
+   float lpx_mem[2 * P];
+   js.lpm = lpx_mem;
+   if constexpr ( JointStore<P, KT>::LPX_MEM ) asm volatile("" : : "v"(lpx_mem) : "memory");
    js.init(L, N);
    JointView<P, KT, 0, 0> va{js};
    JointView<P, KT, KA, 1> vb{js};
@@ -1398,6 +1417,13 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
       CENSUS_LAP(KINC_CENSUS_ESTEP, ct);
       CENSUS_COUNT(KINC_CENSUS_ITERS, (actA ? 1 : 0) + (actB ? 1 : 0));
 
+      /* log p(x) of the next pass-A slot, loaded one slot ahead (the first
+       * before the maxima; the scheduling barriers keep each load where it
+       * is issued) */
+      constexpr bool LPX_MEM = JointStore<P, KT>::LPX_MEM;
+      float lpnA = va.LPX(0), lpnB = vb.LPX(0);
+      if constexpr ( LPX_MEM ) __builtin_amdgcn_sched_barrier(0);
+
       float gmax[KT];
 #pragma unroll
       for ( int k = 0; k < KT; ++k ) gmax[k] = 0.0f;
@@ -1413,8 +1439,15 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
       {
          if ( m < nslots )
          {
-            if ( actA ) joint_terms_a<JointView<P, KT, 0, 0>, KA>(va, m, tile, 0, gmax);
-            if ( actB ) joint_terms_a<JointView<P, KT, KA, 1>, KB>(vb, m, tile, RB, gmax + KA);
+            const float lpA = lpnA, lpB = lpnB;
+            if ( m + 1 < P )
+            {
+               lpnA = va.LPX(m + 1);
+               lpnB = vb.LPX(m + 1);
+               if constexpr ( LPX_MEM ) __builtin_amdgcn_sched_barrier(0);
+            }
+            if ( actA ) joint_terms_a<JointView<P, KT, 0, 0>, KA>(va, m, tile, 0, gmax, lpA);
+            if ( actB ) joint_terms_a<JointView<P, KT, KA, 1>, KB>(vb, m, tile, RB, gmax + KA, lpB);
             CENSUS_COUNT(KINC_CENSUS_A_SLOTS, (actA ? 1 : 0) + (actB ? 1 : 0));
             CENSUS_COUNT(KINC_CENSUS_EXPS, (actA ? 2 * KA : 0) + (actB ? 2 * KB : 0));
             wave_sync(false);

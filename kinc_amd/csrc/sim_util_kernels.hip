@@ -303,6 +303,10 @@ extern "C" int kinc_selftest_math(int32_t fn, const float* in, float* out, int64
  *   OP 0 v_add_f32   1 v_mul_f32   2 v_fma_f32   3 v_fmac_f32   4 v_max_f32
  *   5 v_rndne_f32    6 v_ldexp_f32 7 v_cvt_i32_f32 8 v_cndmask_b32
  *   9 v_fmaak_f32 (FMA with a literal)
+ *   10 one kinc_expf_le0 (the pair kernels' E-step exponential), 11 one
+ *   kinc_expf: compiled code, so the unit is one exponential of a known
+ *   instruction count, and the rate is that of the exponentials' own mix of
+ *   opcodes (acc = exp(-acc) stays in (0.36, 1], no subnormals)
  */
 template <int OP>
 __device__ __forceinline__ void probe_op(float& acc, float inc)
@@ -317,6 +321,8 @@ __device__ __forceinline__ void probe_op(float& acc, float inc)
    if constexpr ( OP == 7 ) asm volatile("v_cvt_i32_f32 %0, %0" : "+v"(acc));
    if constexpr ( OP == 8 ) asm volatile("v_cndmask_b32 %0, %0, %1, vcc" : "+v"(acc) : "v"(inc) : "vcc");
    if constexpr ( OP == 9 ) asm volatile("v_fmaak_f32 %0, %0, %1, 0x3f800000" : "+v"(acc) : "v"(inc));
+   if constexpr ( OP == 10 ) acc = kinc_expf_le0(-acc);
+   if constexpr ( OP == 11 ) acc = kinc_expf(-acc);
 }
 
 template <int CHAINS, int LANES, int OP>
@@ -349,7 +355,7 @@ __global__ __launch_bounds__(64) void k_valu_probe(float* out, int iters)
 static const int kProbeModes[][3] = {
    {8, 64, 0}, {8, 16, 0}, {8, 4, 0}, {1, 64, 0}, {1, 16, 0}, {8, 32, 0},
    {8, 64, 1}, {8, 64, 2}, {8, 64, 3}, {8, 64, 4}, {8, 64, 5}, {8, 64, 6},
-   {8, 64, 7}, {8, 64, 8}, {8, 64, 9}, {1, 64, 2}, {8, 16, 2}};
+   {8, 64, 7}, {8, 64, 8}, {8, 64, 9}, {1, 64, 2}, {8, 16, 2}, {8, 64, 10}, {8, 64, 11}};
 static const int kProbeCount = (int)(sizeof(kProbeModes) / sizeof(kProbeModes[0]));
 
 template <int CH, int LN, int OP>
@@ -368,7 +374,7 @@ extern "C" int kinc_selftest_valu_probe(int32_t mode, int32_t iters, int32_t wor
       probe_fn<1, 16, 0>(), probe_fn<8, 32, 0>(), probe_fn<8, 64, 1>(), probe_fn<8, 64, 2>(),
       probe_fn<8, 64, 3>(), probe_fn<8, 64, 4>(), probe_fn<8, 64, 5>(), probe_fn<8, 64, 6>(),
       probe_fn<8, 64, 7>(), probe_fn<8, 64, 8>(), probe_fn<8, 64, 9>(), probe_fn<1, 64, 2>(),
-      probe_fn<8, 16, 2>()};
+      probe_fn<8, 16, 2>(), probe_fn<8, 64, 10>(), probe_fn<8, 64, 11>()};
    const void* fn = fns[mode];
    float* d_out = nullptr;
    if ( hipMalloc(&d_out, sizeof(float) * 64 * (size_t)workgroups) != hipSuccess ) return -3;
