@@ -163,9 +163,9 @@ def run_form(native, x, flags, form, blocks, warmup, torch, workers=2, autotune=
     tuning = engs[0].tuning()
     if autotune:
         start = blocks[0] * BLOCK
-        tuning = engs[0].autotune(start, min(8192, total_pairs - start))
-        for e in engs[1:]:
-            e.set_tuning(tuning["pair_waves"], tuning["aux_streams"], tuning["joint45"])
+        # one work block run by all worker handles at once, as they share the
+        # device in the timed region
+        tuning = native.autotune_group(engs, start, min(BLOCK, total_pairs - start))
         log("  %s autotune: %s" % (form, tuning))
     MC = p.max_clusters
     nsteps = len(blocks)

@@ -205,6 +205,12 @@ int kinc_sim_profile_read(kinc_sim* h, int32_t* slots, int64_t* launches,
  * checks that all of them give identical K and correlations, keeps the
  * fastest on the handle and reports it (candidates tried, best time of the
  * sample in ms).  Results of later calls are unchanged by it.
+ * kinc_sim_autotune_group does the same for n handles (1..16) created with
+ * the same device, matrix and parameters, such as the worker handles of one
+ * run: each candidate runs the sample on every handle at once, each on its
+ * own stream, is timed from the common start to the last finish, must give
+ * the same bits on every handle, and the fastest is set on all of them
+ * (kinc_sim_autotune is the group of one).  Errors are reported on hs[0].
  */
 typedef struct kinc_sim_tuning
 {
@@ -215,7 +221,11 @@ typedef struct kinc_sim_tuning
    double sample_ms;
 } kinc_sim_tuning;
 
+#define KINC_AUTOTUNE_MAX_HANDLES 16
+
 int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc_sim_tuning* out);
+int kinc_sim_autotune_group(kinc_sim* const* hs, int32_t n, int64_t start, int64_t count,
+                            kinc_sim_tuning* out);
 int kinc_sim_get_tuning(const kinc_sim* h, kinc_sim_tuning* out);
 int kinc_sim_set_tuning(kinc_sim* h, const kinc_sim_tuning* t);
 

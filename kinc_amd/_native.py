@@ -73,6 +73,7 @@ SIGNATURES = [
                                                 ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
     ("kinc_selftest_census", ctypes.c_int, [_P, _P]),
     ("kinc_sim_autotune", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _P]),
+    ("kinc_sim_autotune_group", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _P]),
     ("kinc_sim_get_tuning", ctypes.c_int, [_P, _P]),
     ("kinc_sim_set_tuning", ctypes.c_int, [_P, _P]),
     ("kinc_sim_plan_get", ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P]),
@@ -306,6 +307,22 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def autotune_group(engines, start, count):
+    """Time the launch configurations with every engine of the group running
+    the sample [start, start+count) at once, and set the fastest on all of them
+    (kinc_sim_autotune_group; the engines must share device, matrix and
+    parameters); returns it as a dict."""
+    engines = list(engines)
+    arr = (_P * len(engines))(*[e.h for e in engines])
+    t = Tuning()
+    rc = lib().kinc_sim_autotune_group(ctypes.cast(arr, _P), len(engines), start, count,
+                                       ctypes.byref(t))
+    if rc != KINC_OK:
+        h = engines[0].h if engines else None
+        raise KincError(rc, lib().kinc_sim_last_error(h).decode())
+    return t.as_dict()
 
 
 class Graph:

@@ -12,8 +12,9 @@
  * plus flags the reference does not have: --device N, --form
  * plain|persistent, --pearson-fp64 BOOL (Pearson sums and quotient in
  * double), --workers 1|2 (engine handles taking blocks in turn) and
- * --autotune BOOL (default true: time the launch configurations on a sample
- * first, kinc_sim_autotune; results are the same either way); and the
+ * --autotune BOOL (default true: time the launch configurations on the first
+ * work block with all worker handles, kinc_sim_autotune_group; results are
+ * the same either way); and the
  * expression-matrix converters around it (importexpressionmatrix_input.cpp,
  * exportexpressionmatrix_input.cpp):
  *   kinc run import-emx --input A.txt --output A.emx [--nan TOKEN]
@@ -299,19 +300,19 @@ int main(int argc, char** argv)
          return 1;
       }
    }
-   /* launch configuration: timed on the first block's first 8192 pairs by
-    * the first handle, then used by all (results do not depend on it) */
+   /* launch configuration: timed on the first work block with all worker
+    * handles running it at once, as they share the device later (results do
+    * not depend on it) */
    if ( autotune )
    {
-      const int64_t n = std::min<int64_t>(8192, kinc_pair_count(G));
+      const int64_t n = std::min<int64_t>(KINC_WORK_BLOCK_SIZE, kinc_pair_count(G));
       kinc_sim_tuning t;
-      if ( kinc_sim_autotune(hs[0], 0, n, &t) != KINC_OK )
+      if ( kinc_sim_autotune_group(hs, workers, 0, n, &t) != KINC_OK )
       {
          fprintf(stderr, "%s\n", kinc_sim_last_error(hs[0]));
          for ( int k = 0; k < workers; ++k ) kinc_sim_destroy(hs[k]);
          return 1;
       }
-      for ( int k = 1; k < workers; ++k ) kinc_sim_set_tuning(hs[k], &t);
    }
    kinc_similarity_writer* w = nullptr;
    if ( kinc_similarity_writer_open(ccm.c_str(), cmx.c_str(), G, S, p.max_clusters, p.corr_method,

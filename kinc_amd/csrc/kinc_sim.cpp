@@ -994,46 +994,104 @@ extern "C" const char* kinc_sim_last_error(const kinc_sim* h)
  * CU; 3 where it allows fewer than 16), how many auxiliary streams the
  * persistent form's bins are spread over, and whether the GMM fits with 4
  * and 5 clusters run in lockstep (where the LDS layout holds their tile).
- * kinc_sim_autotune times each on a sample of pairs and keeps the fastest.
+ * kinc_sim_autotune_group times each on a sample of pairs, run by all the
+ * handles of a group at once, and keeps the fastest; kinc_sim_autotune is
+ * the group of one.  Timed alone, one handle of a two-handle run preferred
+ * two auxiliary streams where the pair runs 5% faster with four, so the
+ * handles that will share the device are timed together.
  */
-static int time_config(kinc_sim* h, int64_t start, int count, int waves, int naux, bool j45, double* ms)
+
+/* Time one configuration on a group of handles: every handle runs the
+ * sample on its own stream, all starting together, and the time is that of
+ * the whole group (the worker handles of one run share the device, so a
+ * configuration is judged as they will use it). */
+static int time_config(kinc_sim* const* hs, int n, int64_t start, int count, int waves, int naux,
+                       bool j45, double* ms)
 {
-   apply_config(h, waves, naux, j45);
-   hipEvent_t e0 = nullptr, e1 = nullptr;
-   HIP_TRY(h, hipEventCreate(&e0));
+   kinc_sim* h = hs[0];
+   for ( int i = 0; i < n; ++i ) apply_config(hs[i], waves, naux, j45);
+   std::vector<hipEvent_t> ev((size_t)n + 1, nullptr);
    int rc = KINC_OK;
-   if ( hipEventCreate(&e1) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventCreate failed");
+   for ( size_t i = 0; i < ev.size() && rc == KINC_OK; ++i )
+   {
+      if ( hipEventCreate(&ev[i]) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventCreate failed");
+   }
    double best = 1e30;
    for ( int rep = 0; rep < 3 && rc == KINC_OK; ++rep )
    {
       /* the first run warms up (code objects, caches); the best of the others counts */
-      if ( hipEventRecord(e0, h->stream) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventRecord failed");
-      if ( rc == KINC_OK )
+      if ( hipEventRecord(ev[0], h->stream) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventRecord failed");
+      for ( int i = 1; i < n && rc == KINC_OK; ++i )
       {
-         rc = run(h, start, nullptr, nullptr, count, h->d_K, nullptr, h->d_corr, nullptr, h->stream);
+         if ( hipStreamWaitEvent(hs[i]->stream, ev[0], 0) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipStreamWaitEvent failed");
       }
-      if ( rc == KINC_OK && hipEventRecord(e1, h->stream) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventRecord failed");
-      if ( rc == KINC_OK && hipEventSynchronize(e1) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventSynchronize failed");
+      for ( int i = 0; i < n && rc == KINC_OK; ++i )
+      {
+         rc = run(hs[i], start, nullptr, nullptr, count, hs[i]->d_K, nullptr, hs[i]->d_corr, nullptr, hs[i]->stream);
+         if ( rc && i > 0 ) std::snprintf(h->err, sizeof(h->err), "%s", hs[i]->err);
+      }
+      for ( int i = 1; i < n && rc == KINC_OK; ++i )
+      {
+         if ( hipEventRecord(ev[i], hs[i]->stream) != hipSuccess ||
+              hipStreamWaitEvent(h->stream, ev[i], 0) != hipSuccess )
+         {
+            rc = fail_c(h, KINC_E_DEVICE, "joining the group's streams failed");
+         }
+      }
+      if ( rc == KINC_OK && hipEventRecord(ev[n], h->stream) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventRecord failed");
+      if ( rc == KINC_OK && hipEventSynchronize(ev[n]) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventSynchronize failed");
       float t = 0.0f;
-      if ( rc == KINC_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventElapsedTime failed");
+      if ( rc == KINC_OK && hipEventElapsedTime(&t, ev[0], ev[n]) != hipSuccess ) rc = fail_c(h, KINC_E_DEVICE, "hipEventElapsedTime failed");
       if ( rc == KINC_OK && rep > 0 ) best = std::min(best, (double)t);
    }
-   if ( e0 ) (void)hipEventDestroy(e0);
-   if ( e1 ) (void)hipEventDestroy(e1);
+   for ( hipEvent_t e : ev )
+   {
+      if ( e ) (void)hipEventDestroy(e);
+   }
    *ms = best;
    return rc;
 }
 
-extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc_sim_tuning* out)
+/* handles one configuration means the same on: device, matrix shape and
+ * every parameter (kinc_sim_params holds 4-byte fields only) */
+static bool same_setup(const kinc_sim* a, const kinc_sim* b)
 {
-   if ( !h ) return KINC_E_INVALID;
+   return a->device == b->device && a->G == b->G && a->S == b->S &&
+          std::memcmp(&a->p, &b->p, sizeof(a->p)) == 0;
+}
+
+extern "C" int kinc_sim_autotune_group(kinc_sim* const* hs, int32_t n, int64_t start, int64_t count,
+                                       kinc_sim_tuning* out)
+{
+   if ( !hs || n < 1 || n > KINC_AUTOTUNE_MAX_HANDLES ) return fail(nullptr, KINC_E_INVALID, "autotune: 1..16 handles");
+   for ( int i = 0; i < n; ++i )
+   {
+      if ( !hs[i] ) return fail(nullptr, KINC_E_INVALID, "autotune: null handle");
+   }
+   kinc_sim* h = hs[0];
+   for ( int i = 0; i < n; ++i )
+   {
+      for ( int j = 0; j < i; ++j )
+      {
+         if ( hs[j] == hs[i] ) return fail_c(h, KINC_E_INVALID, "autotune: a handle appears twice");
+      }
+      if ( !same_setup(h, hs[i]) ) return fail_c(h, KINC_E_INVALID, "autotune: the handles differ in device, matrix shape or parameters");
+      if ( hs[i]->capturing ) return fail_c(h, KINC_E_INVALID, "autotune: a handle is recording a graph");
+   }
    if ( count < 1 || count > KINC_WORK_BLOCK_SIZE || count > h->cap )
    {
       return fail_c(h, KINC_E_INVALID, "autotune sample must hold 1..min(32768, batch_capacity) pairs");
    }
    if ( start < 0 || start + count > kinc_pair_count(h->G) ) return fail_c(h, KINC_E_INVALID, "pair range outside the matrix");
-   int rc = ensure_staging(h);
-   if ( rc ) return rc;
+   for ( int i = 0; i < n; ++i )
+   {
+      const int rc = ensure_staging(hs[i]);
+      if ( rc )
+      {
+         if ( i > 0 ) std::snprintf(h->err, sizeof(h->err), "%s", hs[i]->err);
+         return rc;
+      }
+   }
    HIP_TRY(h, hipSetDevice(h->device));
    const int saved_waves = h->waves;
    const int saved_naux = h->naux;
@@ -1046,6 +1104,7 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
    bool best_j45 = saved_j45;
    double best_ms = 1e30;
    int tried = 0;
+   int rc = KINC_OK;
    std::vector<int8_t> K0((size_t)count), K1((size_t)count);
    std::vector<float> C0((size_t)count * (size_t)h->p.max_clusters), C1(C0.size());
    for ( int c = 0; c < 18 && rc == KINC_OK; ++c )
@@ -1055,23 +1114,26 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
       const int naux = nauxs[c % 3];
       if ( (j45 && h->lds_j45 == 0) || !waves_allowed(h, waves, j45) ) continue;
       double ms = 0.0;
-      rc = time_config(h, start, (int)count, waves, naux, j45, &ms);
+      rc = time_config(hs, n, start, (int)count, waves, naux, j45, &ms);
       if ( rc ) break;
-      /* every configuration must give the same bits */
-      std::vector<int8_t>& K = tried == 0 ? K0 : K1;
-      std::vector<float>& C = tried == 0 ? C0 : C1;
-      if ( hipMemcpy(K.data(), h->d_K, (size_t)count, hipMemcpyDeviceToHost) != hipSuccess ||
-           hipMemcpy(C.data(), h->d_corr, sizeof(float) * C.size(), hipMemcpyDeviceToHost) != hipSuccess )
+      /* every configuration and every handle must give the same bits */
+      for ( int i = 0; i < n && rc == KINC_OK; ++i )
       {
-         rc = fail_c(h, KINC_E_DEVICE, "autotune: copying the sample's results failed");
-         break;
+         const bool first = tried == 0 && i == 0;
+         std::vector<int8_t>& K = first ? K0 : K1;
+         std::vector<float>& C = first ? C0 : C1;
+         if ( hipMemcpy(K.data(), hs[i]->d_K, (size_t)count, hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(C.data(), hs[i]->d_corr, sizeof(float) * C.size(), hipMemcpyDeviceToHost) != hipSuccess )
+         {
+            rc = fail_c(h, KINC_E_DEVICE, "autotune: copying the sample's results failed");
+         }
+         else if ( !first && (std::memcmp(K0.data(), K1.data(), K0.size()) != 0 ||
+                              std::memcmp(C0.data(), C1.data(), sizeof(float) * C0.size()) != 0) )
+         {
+            rc = fail_c(h, KINC_E_DEVICE, "autotune: launch configurations or handles gave different results");
+         }
       }
-      if ( tried > 0 && (std::memcmp(K0.data(), K1.data(), K0.size()) != 0 ||
-                         std::memcmp(C0.data(), C1.data(), sizeof(float) * C0.size()) != 0) )
-      {
-         rc = fail_c(h, KINC_E_DEVICE, "autotune: launch configurations gave different results");
-         break;
-      }
+      if ( rc ) break;
       ++tried;
       if ( ms < best_ms )
       {
@@ -1081,12 +1143,11 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
          best_j45 = j45;
       }
    }
-   if ( rc )
-   {
-      apply_config(h, saved_waves, saved_naux, saved_j45);
-      return rc;
-   }
-   apply_config(h, best_waves, best_naux, best_j45);
+   const int w = rc ? saved_waves : best_waves;
+   const int a = rc ? saved_naux : best_naux;
+   const bool j = rc ? saved_j45 : best_j45;
+   for ( int i = 0; i < n; ++i ) apply_config(hs[i], w, a, j);
+   if ( rc ) return rc;
    if ( out )
    {
       out->pair_waves = h->waves;
@@ -1096,6 +1157,12 @@ extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc
       out->sample_ms = best_ms;
    }
    return KINC_OK;
+}
+
+extern "C" int kinc_sim_autotune(kinc_sim* h, int64_t start, int64_t count, kinc_sim_tuning* out)
+{
+   if ( !h ) return KINC_E_INVALID;
+   return kinc_sim_autotune_group(&h, 1, start, count, out);
 }
 
 extern "C" int kinc_selftest_census(kinc_sim* h, uint64_t* d_counters)

@@ -135,3 +135,37 @@ def test_wide_register_builds(S, flags):
     finally:
         eng.close()
     assert_block_equal(got, oracle.execute_block(p, x, 0, n), S, "oracle")
+
+
+def test_group_autotune_sets_one_configuration_on_every_worker():
+    """kinc_sim_autotune_group: the worker handles of one run are timed
+    together and all get the fastest configuration; their results are those
+    of an untuned handle and of the oracle.  Handles that differ in
+    parameters or appear twice are refused."""
+    S = 256
+    flags = dict(clusmethod="gmm", corrmethod="spearman", crit="BIC", preout=True, postout=True)
+    x = gemgen.modules(140, S, seed=36, nan_rate=0.15)
+    p = _native.make_params(**flags)
+    engs = [_native.Engine(x, p) for _ in range(2)]
+    other = _native.Engine(x, _native.make_params(**dict(flags, maxclus=4)))
+    try:
+        want = engs[0].execute(0, 3000)
+        t = _native.autotune_group(engs, 500, 2000)
+        print("group tuning", t, flush=True)
+        # 256 samples: (4-wave, lockstep off/on) and (5-wave, off), 3 stream counts
+        assert t["candidates"] == 9 and t["sample_ms"] > 0
+        for e in engs:
+            got = e.tuning()
+            assert (got["pair_waves"], got["aux_streams"], got["joint45"]) == \
+                (t["pair_waves"], t["aux_streams"], t["joint45"])
+            assert_block_equal(e.execute(0, 3000), want, S, "tuned worker")
+        for bad in ([engs[0], engs[0]], [engs[0], other]):
+            with pytest.raises(_native.KincError) as err:
+                _native.autotune_group(bad, 500, 2000)
+            assert err.value.code == _native.KINC_E_INVALID
+        # a refused group leaves the handles' configuration as it was
+        assert engs[0].tuning()["aux_streams"] == t["aux_streams"]
+    finally:
+        for e in engs + [other]:
+            e.close()
+    assert_block_equal(want, oracle.execute_block(p, x, 0, 3000), S, "oracle")

@@ -110,3 +110,13 @@ def test_rmt_default_params_match_reference(native):
     assert p.threshold_stop == 0.5 and abs(p.chi_square1 - 99.607) < 1e-4
     assert p.chi_square2 == 200 and p.min_eigenvalues == 50
     assert (p.min_pace, p.max_pace, p.bins) == (10, 40, 60)
+
+
+def test_group_autotune_rejects_bad_groups_before_device(native):
+    # no handles, too many, a null handle: refused without touching a device
+    fn = native.lib().kinc_sim_autotune_group
+    null2 = (native._P * 2)(None, None)
+    assert fn(None, 1, 0, 1, None) == native.KINC_E_INVALID
+    assert fn(ctypes.cast(null2, native._P), 0, 0, 1, None) == native.KINC_E_INVALID
+    assert fn(ctypes.cast(null2, native._P), 17, 0, 1, None) == native.KINC_E_INVALID
+    assert fn(ctypes.cast(null2, native._P), 2, 0, 1, None) == native.KINC_E_INVALID
