@@ -57,12 +57,12 @@ WORKLOADS = {
 # 32,443-pair bin of block 97, one handle), from the FETCH_SIZE and
 # WRITE_SIZE passes (scaled by a copy kernel of known size) recorded in:
 TRAFFIC_SOURCE = "profiles/k_pairs_gmm_spearman_20000x500_counters.md"
-TRAFFIC_BYTES_PER_LAUNCH = 8.57e9
+TRAFFIC_BYTES_PER_LAUNCH = 9.98e9
 # VALU wave64 instructions per second of the same dispatch (SQ_INSTS_VALU /
-# its 40.19 ms) against the fastest VALU stream the issue probe measured on
-# the same box (v_add_f32, 4 waves per SIMD; profiles/valu_issue_calibration.md),
+# its 40.68 ms) against the fastest VALU stream the issue probe measured on
+# the same kind of box (v_add_f32, 4 waves per SIMD; profiles/valu_issue_calibration.md),
 # and the lane utilisation SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)
-VALU_ISSUE_RATE = 8.24e11
+VALU_ISSUE_RATE = 8.15e11
 VALU_ISSUE_CEILING = 1.17e12
 VALU_LANE_UTILIZATION = 0.859
 BLOCK = 32768
