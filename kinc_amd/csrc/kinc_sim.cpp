@@ -1093,15 +1093,16 @@ extern "C" int kinc_sim_autotune_group(kinc_sim* const* hs, int32_t n, int64_t s
       }
    }
    HIP_TRY(h, hipSetDevice(h->device));
-   const int saved_waves = h->waves;
-   const int saved_naux = h->naux;
-   const bool saved_j45 = h->base.joint45 != 0;
+   /* each handle's configuration, restored if the tuning fails */
+   struct Saved { int waves, naux; bool j45; };
+   std::vector<Saved> saved((size_t)n);
+   for ( int i = 0; i < n; ++i ) saved[(size_t)i] = {hs[i]->waves, hs[i]->naux, hs[i]->base.joint45 != 0};
    /* candidates; the other builds and the lockstep fits only where the LDS
     * leaves room for them */
    const int wavess[3] = {3, 4, 5};
    const int nauxs[3] = {1, 2, 4};
-   int best_waves = saved_waves, best_naux = saved_naux;
-   bool best_j45 = saved_j45;
+   int best_waves = saved[0].waves, best_naux = saved[0].naux;
+   bool best_j45 = saved[0].j45;
    double best_ms = 1e30;
    int tried = 0;
    int rc = KINC_OK;
@@ -1143,11 +1144,15 @@ extern "C" int kinc_sim_autotune_group(kinc_sim* const* hs, int32_t n, int64_t s
          best_j45 = j45;
       }
    }
-   const int w = rc ? saved_waves : best_waves;
-   const int a = rc ? saved_naux : best_naux;
-   const bool j = rc ? saved_j45 : best_j45;
-   for ( int i = 0; i < n; ++i ) apply_config(hs[i], w, a, j);
-   if ( rc ) return rc;
+   if ( rc )
+   {
+      for ( int i = 0; i < n; ++i )
+      {
+         apply_config(hs[i], saved[(size_t)i].waves, saved[(size_t)i].naux, saved[(size_t)i].j45);
+      }
+      return rc;
+   }
+   for ( int i = 0; i < n; ++i ) apply_config(hs[i], best_waves, best_naux, best_j45);
    if ( out )
    {
       out->pair_waves = h->waves;
