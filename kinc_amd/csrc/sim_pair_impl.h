@@ -1418,8 +1418,8 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
       CENSUS_COUNT(KINC_CENSUS_ITERS, (actA ? 1 : 0) + (actB ? 1 : 0));
 
       /* log p(x) of the next pass-A slot, loaded one slot ahead (the first
-       * before the maxima; the scheduling barriers keep each load where it
-       * is issued) */
+       * before the maxima; nslots >= 1; the scheduling barriers keep each
+       * load where it is issued) */
       constexpr bool LPX_MEM = JointStore<P, KT>::LPX_MEM;
       float lpnA = va.LPX(0), lpnB = vb.LPX(0);
       if constexpr ( LPX_MEM ) __builtin_amdgcn_sched_barrier(0);
@@ -1440,7 +1440,7 @@ __device__ __forceinline__ void gmm_fit_joint(const Lds& L, int N, bool want_ent
          if ( m < nslots )
          {
             const float lpA = lpnA, lpB = lpnB;
-            if ( m + 1 < P )
+            if ( m + 1 < nslots )   /* only slots the E step wrote */
             {
                lpnA = va.LPX(m + 1);
                lpnB = vb.LPX(m + 1);
