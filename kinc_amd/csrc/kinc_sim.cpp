@@ -420,10 +420,11 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       }
       else if ( a.clus == 2 )
       {
-         /* the mean chains' 2K rows, the log-likelihood terms (N floats) */
+         /* the mean chains' 2K rows; the log-likelihood terms pass through
+          * the same region in chunks of a.ll_cap */
          shared = std::max(shared, 2 * (size_t)h->kmax * row);
-         shared = std::max(shared, sizeof(float) * (size_t)S);
       }
+      a.ll_cap = (int32_t)((shared / sizeof(float)) & ~(size_t)3);
       h->lds_small = align16(off + shared);
       h->lds_j45 = 0;
       if ( a.clus == 1 && a.max_k >= 5 && a.min_k <= 4 )
@@ -456,6 +457,7 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       off = align16(off + (size_t)S);
       a.lds_u = (int32_t)off;
       off = align16(off + sort_bytes);
+      a.ll_cap = 2 * a.np2;   /* the sort buffers: room for every term */
       a.scr_lg = (int32_t)off;
       /* per-point responsibilities and log p(x); the run-time-K fits keep
        * log p(x) only and recompute the rest */

@@ -105,6 +105,9 @@ struct SimArgs
    int32_t mem_slots;        /* ceil(S / 64) */
    int32_t np2;              /* next power of two >= S (min 2) */
    int32_t tile_stride;      /* KINC_TILE_STRIDE */
+   int32_t ll_cap;           /* floats at lds_u the K-means log-likelihood terms
+                                may fill before they are added (a multiple of 4,
+                                at least 64) */
    unsigned long long* census;  /* phase-census build only: its counters */
 };
 

@@ -1813,14 +1813,39 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
       }
 
       /* computeLogLikelihood (pairwise_kmeans.cpp:106-127): one chain over
-       * the members in (k, i) order */
+       * the members in (k, i) order.  The terms are staged in order in the
+       * shared region (the mean chains' tile is free here) and added
+       * whenever the next slot's members would not fit a.ll_cap values, so
+       * the region need not hold all N of them; the adds run in the same
+       * order either way. */
       float* seq = L.u;
-      int kbase = 0;
+      const int cap = a.ll_cap;
+      int pos = 0;
+      float Ssum = 0.0f;
+      auto add_staged = [&]() {
+         wave_sync(L.lgbuf != nullptr);
+         if ( lane < KINC_MIN_ACTIVE_LANES )   /* identical sums; lane 0's is used */
+         {
+            /* four values per LDS read, added in index order */
+            const float4* s4 = (const float4*)seq;
+            const int n4 = pos >> 2;
+            for ( int j = 0; j < n4; ++j )
+            {
+               const float4 v = s4[j];
+               Ssum = Ssum + v.x;
+               Ssum = Ssum + v.y;
+               Ssum = Ssum + v.z;
+               Ssum = Ssum + v.w;
+            }
+            for ( int j = n4 << 2; j < pos; ++j ) Ssum = Ssum + seq[j];
+         }
+         wave_sync(L.lgbuf != nullptr);
+         pos = 0;
+      };
 #pragma unroll
       for ( int k = 0; k < K; ++k )
       {
          const float2 mk = *(const float2*)(means + 2 * k);
-         int cnt = 0;
 #pragma unroll
          for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
          {
@@ -1828,35 +1853,19 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
             {
                const bool mem = (lane + m * WAVE < N) && st.Y(m) == k;
                const uint64_t bm = ballot(mem);
+               const int c = popc(bm);
+               if ( pos + c > cap ) add_staged();   /* cap >= 64 >= c */
                if ( mem )
                {
                   const float d = diff_norm(st.X0(m), st.X1(m), mk.x, mk.y);
-                  seq[kbase + cnt + popc(bm & lanes_below())] = d * d;
+                  seq[pos + popc(bm & lanes_below())] = d * d;
                }
-               cnt += popc(bm);
+               pos += c;
             }
          }
-         kbase += cnt;
       }
-      wave_sync(L.lgbuf != nullptr);
-      float Ssum = 0.0f;
-      if ( lane < KINC_MIN_ACTIVE_LANES )   /* identical sums; lane 0's is used */
-      {
-         /* four values per LDS read, added in index order */
-         const float4* s4 = (const float4*)seq;
-         const int n4 = kbase >> 2;
-         for ( int j = 0; j < n4; ++j )
-         {
-            const float4 v = s4[j];
-            Ssum = Ssum + v.x;
-            Ssum = Ssum + v.y;
-            Ssum = Ssum + v.z;
-            Ssum = Ssum + v.w;
-         }
-         for ( int j = n4 << 2; j < kbase; ++j ) Ssum = Ssum + seq[j];
-      }
+      add_staged();
       const float logL = -bcast(Ssum, 0);
-      wave_sync(L.lgbuf != nullptr);
 
       if ( bestLogL < logL )
       {
