@@ -1783,19 +1783,19 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
                   rank = (ym == k) ? popc(bm & lanes_below()) : rank;
                }
                const int len = (maxc + 3) & ~3;
+               /* every row zeroed by all lanes, then the members written
+                * over the front (the wave's LDS stores land in order), so
+                * each row holds its members and zeros up to len */
+#pragma unroll
+               for ( int k = 0; k < K; ++k )
+               {
+                  tile[(2 * k) * STR + lane] = 0.0f;
+                  tile[(2 * k + 1) * STR + lane] = 0.0f;
+               }
                if ( ym >= 0 )
                {
                   tile[(2 * ym) * STR + rank] = xa;
                   tile[(2 * ym + 1) * STR + rank] = xb;
-               }
-#pragma unroll
-               for ( int k = 0; k < K; ++k )
-               {
-                  if ( lane >= ck[k] && lane < len )
-                  {
-                     tile[(2 * k) * STR + lane] = 0.0f;
-                     tile[(2 * k + 1) * STR + lane] = 0.0f;
-                  }
                }
                wave_sync(L.lgbuf != nullptr);
                acc = chain_add4(tile, STR, 2 * K, len, acc);
