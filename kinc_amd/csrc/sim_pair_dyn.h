@@ -349,18 +349,18 @@ __device__ __forceinline__ int dyn_nearest(const float* means, int K, float xa, 
 {
    int kq = 0;
    float qmin = diff_sq(xa, xb, means[0], means[1]);
+   float before = NAN;   /* smallest q before kq, as in nearest_mean_q */
    for ( int k = 1; k < K; ++k )
    {
       const float q = diff_sq(xa, xb, means[2 * k], means[2 * k + 1]);
       if ( q < qmin )
       {
+         before = qmin;
          kq = k;
          qmin = q;
       }
    }
-   const float thr = qmin * (1.0f + 0x1p-20f);
-   bool tie = false;
-   for ( int k = 0; k < kq; ++k ) tie = tie || (diff_sq(xa, xb, means[2 * k], means[2 * k + 1]) <= thr);
+   const bool tie = before <= qmin * (1.0f + 0x1p-20f);
    if ( ballot(tie) != 0 )
    {
       /* the reference's comparison of square roots */

@@ -1641,29 +1641,29 @@ __device__ __forceinline__ float diff_sq(float ax, float ay, float bx, float by)
  * monotone and correctly rounded, so the first k with the smallest q_k (same
  * comparisons on q) is that k unless an earlier k has a q_k so close above
  * the minimum that both square roots round to the same float, which needs
- * q_k < q_min * (1 + 2^-21).  Such a near tie is flagged (q_k <= q_min *
- * (1 + 2^-20), a safe margin also in the subnormal range) and the caller then
- * decides with the square roots.
+ * q_k < q_min * (1 + 2^-20).  Such a near tie is flagged (some earlier q_k <=
+ * q_min * (1 + 2^-20), a safe margin also in the subnormal range) and the
+ * caller then decides with the square roots.  The smallest q_k before the
+ * chosen k is the running minimum at the moment the choice last moved, so
+ * the flag needs one comparison, not one per earlier k.
  */
 template <int K>
 __device__ __forceinline__ int nearest_mean_q(const float (&q)[K], bool& near_tie)
 {
    int kq = 0;
    float qmin = q[0];
+   float before = NAN;   /* smallest q_k with k < kq; NaN (no flag) while kq = 0 */
 #pragma unroll
    for ( int k = 1; k < K; ++k )
    {
       if ( q[k] < qmin )
       {
+         before = qmin;
          kq = k;
          qmin = q[k];
       }
    }
-   const float thr = qmin * (1.0f + 0x1p-20f);
-   bool tie = false;
-#pragma unroll
-   for ( int k = 0; k < K - 1; ++k ) tie = tie || (k < kq && q[k] <= thr);
-   near_tie = tie;
+   near_tie = before <= qmin * (1.0f + 0x1p-20f);
    return kq;
 }
 
