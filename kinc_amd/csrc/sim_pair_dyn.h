@@ -447,6 +447,10 @@ __device__ __noinline__ FitOut kmeans_fit_dyn(const SimArgs& a, const Lds& L, in
                if ( g >= groups ) break;
                const int k0 = 32 * g;
                const int k1 = k0 + 32 < K ? k0 + 32 : K;
+               /* the group's rows zeroed by all lanes, then each member
+                * written over the front (the wave's LDS stores land in
+                * order): every row holds its members, then zeros */
+               for ( int r = 0; r < 2 * (k1 - k0); ++r ) tile[r * STR + lane] = 0.0f;
                int maxc = 0;
                for ( int k = k0; k < k1; ++k )
                {
@@ -461,18 +465,7 @@ __device__ __noinline__ FitOut kmeans_fit_dyn(const SimArgs& a, const Lds& L, in
                      tile[(2 * (k - k0) + 1) * STR + rank] = xb;
                   }
                }
-               /* every row of the group filled up to the common length */
                const int len = (maxc + 3) & ~3;
-               for ( int k = k0; k < k1; ++k )
-               {
-                  const uint64_t bm = ballot(ym == k);
-                  const int ck = popc(bm);
-                  if ( lane >= ck && lane < len )
-                  {
-                     tile[(2 * (k - k0)) * STR + lane] = 0.0f;
-                     tile[(2 * (k - k0) + 1) * STR + lane] = 0.0f;
-                  }
-               }
                wave_sync(true);
                acc[g] = chain_add4(tile, STR, 2 * (k1 - k0), len, acc[g]);
                wave_sync(true);
