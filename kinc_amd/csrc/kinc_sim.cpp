@@ -464,7 +464,7 @@ static void setup_layout(kinc_sim* h, const kinc_sim_params* params, int G, int 
       const size_t lg_rows = h->kmax > 8 ? 1 : (size_t)(h->kmax + 1);
       off = align16(off + sizeof(float) * lg_rows * (size_t)a.mem_slots * 64);
       a.scr_y = (int32_t)off;
-      off = align16(off + (size_t)2 * (size_t)a.mem_slots * 64);
+      off = align16(off + (size_t)a.mem_slots * 64);
       a.scratch_stride = (int64_t)((off + 255) & ~(size_t)255);
       a.lds_prm = 0;
       if ( h->kmax > 8 )

@@ -101,7 +101,7 @@ struct SimArgs
    char* scratch;            /* memory-store instance: slices of scratch_stride bytes */
    int64_t scratch_stride;
    int32_t scr_lg;           /* float[(KMAX+1)][mem_slots][64] responsibilities, log p(x) */
-   int32_t scr_y;            /* int8[2][mem_slots][64] K-means labels */
+   int32_t scr_y;            /* int8[mem_slots][64] K-means labels */
    int32_t mem_slots;        /* ceil(S / 64) */
    int32_t np2;              /* next power of two >= S (min 2) */
    int32_t tile_stride;      /* KINC_TILE_STRIDE */

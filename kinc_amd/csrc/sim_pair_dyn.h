@@ -389,7 +389,6 @@ __device__ __noinline__ FitOut kmeans_fit_dyn(const SimArgs& a, const Lds& L, in
    const int lane = lane_id();
    constexpr int STR = KINC_TILE_STRIDE;
    const int nslots = (N + WAVE - 1) / WAVE;
-   const int ms = a.mem_slots;
    float* tile = L.tile;
    float* means = L.prm;
    const int rows = 2 * K;
@@ -401,7 +400,6 @@ __device__ __noinline__ FitOut kmeans_fit_dyn(const SimArgs& a, const Lds& L, in
    float bestLogL = -INFINITY;
    int iters = 0;
    auto Y = [&](int m) -> int8_t& { return L.ybuf[((size_t)m << 6) + lane]; };
-   auto YN = [&](int m) -> int8_t& { return L.ybuf[((size_t)(ms + m) << 6) + lane]; };
 
    for ( int init = 0; init < 10; ++init )
    {
@@ -423,11 +421,12 @@ __device__ __noinline__ FitOut kmeans_fit_dyn(const SimArgs& a, const Lds& L, in
          for ( int m = 0; m < nslots; ++m )
          {
             const int min_k = dyn_nearest(means, K, dyn_x0(L, N, m), dyn_x1(L, N, m));
-            YN(m) = (int8_t)min_k;
-            diff = diff || ((lane + m * WAVE < N) && (int8_t)min_k != Y(m));
+            /* new labels over the old ones, as in kmeans_fit */
+            const int8_t old = Y(m);
+            Y(m) = (int8_t)min_k;
+            diff = diff || ((lane + m * WAVE < N) && (int8_t)min_k != old);
          }
          if ( ballot(diff) == 0 ) break;
-         for ( int m = 0; m < nslots; ++m ) Y(m) = YN(m);
          wave_sync(true);
 
          /* means: rows 2k, 2k+1 over the members of cluster k in index

@@ -559,7 +559,7 @@ struct RegStore
    float x0[P], x1[P];
    float lg[K][P];
    float lpx[P];
-   int8_t y[P], yn[P];
+   int8_t y[P];
 
    __device__ __forceinline__ void init(const Lds& L, int N)
    {
@@ -578,7 +578,6 @@ struct RegStore
    __device__ __forceinline__ float& LG(int k, int m) { return lg[k][m]; }
    __device__ __forceinline__ float& LPX(int m) { return lpx[m]; }
    __device__ __forceinline__ int8_t& Y(int m) { return y[m]; }
-   __device__ __forceinline__ int8_t& YN(int m) { return yn[m]; }
 };
 
 template <int K>
@@ -612,7 +611,6 @@ struct MemStore
    __device__ __forceinline__ float& LG(int k, int m) { return lgb[((size_t)(k * ms + m) << 6) + lane_id()]; }
    __device__ __forceinline__ float& LPX(int m) { return lgb[((size_t)(K * ms + m) << 6) + lane_id()]; }
    __device__ __forceinline__ int8_t& Y(int m) { return yb[((size_t)m << 6) + lane_id()]; }
-   __device__ __forceinline__ int8_t& YN(int m) { return yb[((size_t)(ms + m) << 6) + lane_id()]; }
 };
 
 /* loop bound over slots: the compile-time slot count of a register store
@@ -1742,16 +1740,16 @@ __device__ __forceinline__ FitOut kmeans_fit(const SimArgs& a, const Lds& L, int
                      }
                   }
                }
-               st.YN(m) = (int8_t)min_k;
-               diff = diff || ((lane + m * WAVE < N) && (int8_t)min_k != st.Y(m));
+               /* the new label goes straight over the old one: if none
+                * changed, the loop stops with the labels it had (the
+                * reference's y == y_next); else they are the reference's
+                * swapped-in y_next */
+               const int8_t old = st.Y(m);
+               st.Y(m) = (int8_t)min_k;
+               diff = diff || ((lane + m * WAVE < N) && (int8_t)min_k != old);
             }
          }
          if ( ballot(diff) == 0 ) break;
-#pragma unroll
-         for ( int m = 0; m < slot_bound<ST>(nslots); ++m )
-         {
-            if ( m < nslots ) st.Y(m) = st.YN(m);
-         }
 
          /* means: chain 2k / 2k+1 over the members of cluster k in index
           * order.  Each slot's members of k are packed to the front of rows
